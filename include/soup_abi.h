@@ -1,0 +1,227 @@
+/*
+ * soup_abi.h -- C-ABI of the MI355X soup Langevin-step library (libsoup_hip.so).
+ *
+ * The library owns one simulated periodic box per handle: particle state in HBM, the cell grid,
+ * the Verlet and Debye-Hueckel neighbour lists, and one HIP stream. Every call blocks until the
+ * device work it issued is finished. Calls on one handle must not overlap (the host keeps a
+ * per-handle lock); calls on different handles are independent.
+ *
+ * Every entry point returns 0 on success and a negative SOUP_E* code on failure; the message of the
+ * last failure on a handle is returned by soup_last_error(handle) (soup_last_error(NULL) gives the
+ * message of the last failed soup_create on this thread). Messages keep the stems the reference
+ * throws, so callers that match on them keep working.
+ *
+ * Array conventions at this boundary are the reference's: particle i is the i-th particle of the
+ * creation composition; positions/velocities/forces are 4 floats per particle (x, y, z, w) with
+ * pos.w = the species kind index as a float; bond slots are 3 u32 per particle, 0xFFFFFFFF = none.
+ * Internally the library keeps particles in cell-sorted order and translates at this boundary.
+ */
+#ifndef SOUP_ABI_H
+#define SOUP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOUP_NONE 0xFFFFFFFFu
+#define SOUP_MAX_KINDS 8
+#define SOUP_MAX_RULES 4
+#define SOUP_CLASSES 4
+
+enum {
+  SOUP_OK = 0,
+  SOUP_E_INVALID = -1,     /* bad argument / configuration */
+  SOUP_E_DEVICE = -2,      /* HIP runtime failure */
+  SOUP_E_NONFINITE = -3,   /* non-finite positions/velocities (soup/src/soup-health.ts:59-62) */
+  SOUP_E_OVERFLOW = -4,    /* Verlet or DH list capacity exceeded (soup/src/soup-grid-verlet.ts:95-136) */
+  SOUP_E_DRIFT = -5,       /* measured drift > skin/2 (soup/src/soup-grid-verlet.ts:125-136) */
+  SOUP_E_NOMEM = -6        /* allocation refused (soup/src/sim.ts:296-314) */
+};
+
+/* Resolved, flat parameters. The host resolves CreateSoupOpts + data/soup.json + data/params.json
+ * exactly as soup/src/sim.ts:96-278 does and fills this. f32 fields carry the values the reference
+ * uploads as f32 uniforms (Params forces.wgsl:1-10, GridDims soup-buffers.ts:295-310, Species
+ * pair.wgsl:61, AttrScale soup-attraction.ts:190-205, ES/ES2 electrostatics.ts:253-274,
+ * VL soup-buffers.ts:231-232, BondParams bond-common.wgsl:54-81); f64 fields carry the doubles the
+ * reference's CPU constant-pH Monte Carlo uses (soup/src/electrostatics.ts:435-496). */
+typedef struct SoupConfig {
+  uint32_t n;                 /* live particle count */
+  uint32_t n_kinds;           /* monomer kinds (<= 8) */
+  float box[3];               /* live box, sigma */
+  uint32_t dims[3];           /* grid cells per axis = floor(box/cellSize) */
+  uint32_t walk_radius;       /* GB.dims.w: +-cells walked by the list build */
+  float list_range;           /* VL.x = interactionRange + skin */
+  uint32_t list_cap;          /* VL.y = Verlet capacity per particle */
+  uint32_t rebuild_every;     /* verletList.rebuildEvery */
+  float skin;                 /* verletList.skin */
+  /* Params (forces.wgsl:1-10), kT already overridden by CreateSoupOpts.kT (sim.ts:107) */
+  float sigma, epsilon, k_fene, r_inf, k_bend, r_bend, wc, kT, gamma, dt;
+  float b_tt;                 /* sigma * beadSizes.tail_tail: attraction rc = wca_cut(b_tt) */
+  float wca_factor;           /* f32(2^(1/6)): wca_cut(b) = wca_factor * b */
+  float radius[SOUP_MAX_KINDS];     /* Species.radius */
+  uint32_t cls[SOUP_MAX_KINDS];     /* speciesClass: 0 apolar, 1 polar, 2 solvent, 3 mineral */
+  float attr[SOUP_CLASSES][SOUP_CLASSES]; /* AttrScale.rows */
+  float acid_soap;            /* AttrScale.pair.x */
+  /* Debye-Hueckel, f32 as uploaded (ES, ES2) */
+  uint32_t es_enabled;
+  float es_A, es_kappa, es_rc, es_Frc;
+  float es_split;             /* ES2.x splitRadius */
+  float es_list_range;        /* ES2.y rc_es + skin */
+  uint32_t es_cap;            /* ES2.z per-head capacity */
+  uint32_t charged_kind;      /* ES2.w */
+  uint32_t n_heads;           /* creation census of the charged kind (headIdx size) */
+  /* constant-pH Monte Carlo, f64 (EsBasis) */
+  double ph_A, ph_kappa, ph_cutoff, ph_shiftU, ph_shiftF;
+  double ph_pKa, ph_pH, ph_kT, ph_q;   /* ph_q = chargeDeprotonated */
+  double as_scale, as_rc, as_wc, as_eps; /* AcidSoapWell */
+  uint32_t sweep_every;       /* electrostatics.sweepEverySteps */
+  /* bond Monte Carlo (BondParams) */
+  uint32_t n_rules;
+  float kindA[SOUP_MAX_RULES], kindB[SOUP_MAX_RULES];
+  float att_form[SOUP_MAX_RULES], att_break[SOUP_MAX_RULES];
+  float acc_form[SOUP_MAX_RULES], acc_break[SOUP_MAX_RULES];
+  float req_cat[SOUP_MAX_RULES], roleA[SOUP_MAX_RULES], roleB[SOUP_MAX_RULES];
+  float catalyst_kind, head_terminal_only, head_chain_cap;
+  float max_hold_steps;       /* adsorptionParams.x */
+  float bond_interval_f;      /* adsorptionParams.y */
+  uint32_t bond_interval;     /* bondAttemptInterval.steps */
+  uint32_t reserved[7];
+} SoupConfig;
+
+/* Mutable state at creation: a fresh lattice built by the host (soup/src/soup-init-state.ts:58-307)
+ * or a decoded checkpoint (soup/src/checkpoint.ts:220-237). All arrays have n (or 3n/4n) entries and
+ * are copied in. */
+typedef struct SoupInitState {
+  const float* positions;      /* 4n */
+  const float* velocities;     /* 4n */
+  const uint32_t* bond_slots;  /* 3n */
+  const uint32_t* center_link; /* n */
+  const uint32_t* center_held; /* n */
+  const uint32_t* frozen;      /* n */
+  const uint32_t* bond_rng;    /* n */
+  const uint32_t* thermo_rng;  /* n */
+  const float* charges;        /* n */
+  const uint32_t* events;      /* 2 * n_rules */
+  const uint32_t* desorb_events; /* 2 */
+  uint64_t global_step;
+  uint64_t next_sweep_at;      /* UINT64_MAX: never */
+  uint32_t protonation_rng;
+  uint32_t reserved;
+} SoupInitState;
+
+typedef struct SoupHandle SoupHandle;
+
+/* Fields for soup_read / soup_write. Reads return the reference layout in particle-index order. */
+enum {
+  SOUP_FIELD_POS = 0,          /* f32 x4 */
+  SOUP_FIELD_VEL = 1,          /* f32 x4 */
+  SOUP_FIELD_FORCE = 2,        /* f32 x4: the force buffer as it stands (forcesNoRebuildDEBUG) */
+  SOUP_FIELD_BOND_SLOTS = 3,   /* u32 x3 */
+  SOUP_FIELD_CENTER_LINK = 4,  /* u32 */
+  SOUP_FIELD_CENTER_HELD = 5,  /* u32 */
+  SOUP_FIELD_FROZEN = 6,       /* u32 */
+  SOUP_FIELD_CHARGE = 7,       /* f32 */
+  SOUP_FIELD_BOND_RNG = 8,     /* u32 */
+  SOUP_FIELD_THERMO_RNG = 9,   /* u32 */
+  SOUP_FIELD_EVENTS = 10,      /* u32 x (2 * n_rules) */
+  SOUP_FIELD_DESORB = 11,      /* u32 x2: stretch, timeout */
+  SOUP_FIELD_VERLET_COUNT = 12,/* u32 per particle */
+  SOUP_FIELD_VERLET_LIST = 13, /* u32 x list_cap per particle, entries are particle indices */
+  SOUP_FIELD_ES_COUNT = 14,    /* u32 per head (head order = ascending particle index) */
+  SOUP_FIELD_ES_LIST = 15,     /* u32 x es_cap per head, entries are particle indices */
+  SOUP_FIELD_CELL_OF = 16,     /* u32 per particle: linear cell at the last rebuild */
+  SOUP_FIELD_CELL_START = 17,  /* u32 x (ncells + 1) */
+  SOUP_FIELD_CELL_IDX = 18,    /* u32 x n: particle indices in cell-sorted order */
+  SOUP_FIELD_MAX_DRIFT_SQ = 19,/* f32 x1 */
+  SOUP_FIELD_HEAD_IDX = 20     /* u32 x n_heads */
+};
+
+/* Modes for soup_forces (soup/src/soup-readback.ts:32-65). */
+enum {
+  SOUP_FORCES_REBUILD = 0,     /* full grid/list rebuild, then list force (+ DH far) -> forces() */
+  SOUP_FORCES_BRUTE = 1,       /* O(N^2) force + O(N^2) DH far -> forcesBruteForce() */
+  SOUP_FORCES_LIST = 2         /* list force with the current lists, no rebuild */
+};
+
+/* Single phases of one integration step, for parity tests (soup/src/soup-integrate.ts:83-111). */
+enum {
+  SOUP_PHASE_KICK_DRIFT = 1,   /* kick_drift_wrap_main */
+  SOUP_PHASE_REBUILD = 2,      /* encodeVerletRebuild: grid, Verlet list, DH list, snapshot */
+  SOUP_PHASE_FORCE = 4,        /* soup_force_list_main + es_force_far */
+  SOUP_PHASE_DRIFT = 8,        /* soup_max_drift_main */
+  SOUP_PHASE_BONDS = 16,       /* bond formation + break */
+  SOUP_PHASE_KICK_THERMO = 32  /* kick_thermostat_main */
+};
+
+/* Constant-pH sweep statistics (soup/src/electrostatics.ts:317-326). */
+typedef struct SoupSweepResult {
+  uint32_t attempts, accepted, heads, deprotonated;
+  double alpha, dEsMeanKT;
+} SoupSweepResult;
+
+/* Verlet / DH list occupancy (listOccupancyDEBUG, soup/src/soup-readback.ts:296-377). */
+typedef struct SoupListStats {
+  double verlet_mean; uint32_t verlet_max; uint32_t verlet_cap;
+  double es_mean; uint32_t es_max; uint32_t es_cap;
+  uint64_t verlet_pairs;       /* sum of verletCount */
+  uint64_t es_pairs;           /* sum of esCount */
+} SoupListStats;
+
+/* createSoup (soup/src/sim.ts:95-600) below the host's planning: allocates device state, copies the
+ * initial state in, and runs the priming rebuild + force pass (sim.ts:396-408). */
+int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** out);
+/* dispose (soup/src/soup-buffers.ts:557-596). */
+int soup_destroy(SoupHandle* h);
+/* Text of the last failure; never NULL. */
+const char* soup_last_error(const SoupHandle* h);
+
+/* SoupSystem.step(n) (soup/src/soup-integrate.ts:136-175): n integration steps in chunks of 1000,
+ * with the non-finite and Verlet guards and the due constant-pH sweep at each chunk end. */
+int soup_step(SoupHandle* h, uint32_t n);
+/* Global step counter (SoupSystem.steps). */
+int soup_global_step(const SoupHandle* h, uint64_t* out);
+
+/* Readbacks (soup/src/soup-readback.ts) and writes; bytes must equal the field size. */
+int soup_read(SoupHandle* h, int field, void* dst, size_t bytes);
+/* Writable: POS, VEL, CHARGE, BOND_SLOTS, CENTER_LINK, CENTER_HELD, BOND_RNG, THERMO_RNG. */
+int soup_write(SoupHandle* h, int field, const void* src, size_t bytes);
+/* Size in bytes of a field (0 for an unknown field). */
+size_t soup_field_bytes(const SoupHandle* h, int field);
+
+/* forces() / forcesBruteForce() / list force (soup/src/soup-readback.ts:32-65); the result is in
+ * SOUP_FIELD_FORCE afterwards. */
+int soup_forces(SoupHandle* h, int mode);
+/* One or more single phases, in step order, without advancing the step counter (test hook). */
+int soup_run_phases(SoupHandle* h, uint32_t phase_mask, uint32_t do_bonds_flag);
+
+/* protonationSweepDEBUG / maybeProtonationSweep (soup/src/soup-protonation.ts:52-77): sets the next
+ * due step to now + sweepEverySteps, runs the serial constant-pH chain on the device, then the list
+ * rebuild + force recompute. A no-op (zeroed result) when electrostatics is off. */
+int soup_protonation_sweep(SoupHandle* h, SoupSweepResult* out);
+/* Number of sweeps this handle ran and the statistics of the last one (electrostatics().sweeps/last). */
+int soup_last_sweep(const SoupHandle* h, uint32_t* sweeps, SoupSweepResult* last);
+/* protonationRngState. */
+int soup_protonation_rng(const SoupHandle* h, uint32_t* out);
+
+/* nonFiniteCount (soup/src/soup-health.ts:34-63): out[0] = positions, out[1] = velocities. */
+int soup_nonfinite(SoupHandle* h, uint32_t out[2]);
+/* assertVerletSafety (soup/src/soup-grid-verlet.ts:91-138) on demand. */
+int soup_check_lists(SoupHandle* h);
+/* List occupancy of the current lists. */
+int soup_list_stats(SoupHandle* h, SoupListStats* out);
+
+/* relaxIterations (soup/src/soup-relax.ts:87-139): normalised steepest descent with the cap
+ * decaying linearly from d0 to 0, a rebuild before every iteration, and a final rebuild + force. */
+int soup_relax(SoupHandle* h, uint32_t iterations, float d0);
+
+/* Library build/version string. */
+const char* soup_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOUP_ABI_H */

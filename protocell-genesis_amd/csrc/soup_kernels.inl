@@ -1,0 +1,1247 @@
+// soup_kernels.inl -- HIP kernels of the soup Langevin step for gfx950 (included once by
+// soup_lib.hip). Each kernel cites the reference WGSL entry point it replaces.
+//
+// Exact kernels (cell assignment, list builds, integrator, bond MC, constant-pH chain) use
+// IEEE f32 division/sqrt and no contraction so their results are bit-identical to the oracle.
+// The per-step force kernels are free to reassociate: their results are judged against an FP64
+// recomputation.
+#pragma once
+
+#include "soup_device.h"
+
+#define WAVE 64
+
+// ================================================================================================
+// Small utilities
+// ================================================================================================
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, WAVE);
+  return v;
+}
+
+// ---- exclusive scan of u32 (3 kernels): out[0..m] with out[m] = total -----------------------
+#define SCAN_BLOCK 1024
+#define SCAN_ITEMS 4
+#define SCAN_TILE (SCAN_BLOCK * SCAN_ITEMS)
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+  uint32_t x = v;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint32_t y = (uint32_t)__shfl_up((int)x, o, WAVE);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == WAVE - 1) s_wave[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t s = lane < nw ? s_wave[lane] : 0u;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      uint32_t y = (uint32_t)__shfl_up((int)s, o, WAVE);
+      if (lane >= (uint32_t)o) s += y;
+    }
+    if (lane < nw) s_wave[lane] = s;
+  }
+  __syncthreads();
+  uint32_t base = w > 0 ? s_wave[w - 1] : 0u;
+  total = s_wave[nw - 1];
+  __syncthreads();
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* in, uint32_t m, uint32_t* block_sums) {
+  __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < m) v += in[base + k];
+  uint32_t total;
+  (void)block_excl_scan(v, s_wave, total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_blocks(uint32_t* block_sums, uint32_t nb) {
+  // single block, exclusive scan in place over nb <= SCAN_TILE entries; block_sums[nb] = total
+  __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+  uint32_t base = threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = base + k < nb ? block_sums[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t off = block_excl_scan(sum, s_wave, total);
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < nb) block_sums[base + k] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0) block_sums[nb] = total;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint32_t* in, uint32_t m, const uint32_t* block_sums,
+                                                           uint32_t nb, uint32_t* out) {
+  __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = base + k < m ? in[base + k] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t off = block_excl_scan(sum, s_wave, total) + block_sums[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < m) out[base + k] = off;
+    off += v[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = block_sums[nb];
+}
+
+// ================================================================================================
+// Integrator: kick_drift_wrap_main (soup/wgsl/step.wgsl:96-108),
+//             kick_thermostat_main (step.wgsl:110-142)
+// ================================================================================================
+__global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  if (b.frozen[t] != 0u) {
+    b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  float4 v = b.vel[t], f = b.force[t], x = b.pos[t];
+  float hdt = 0.5f * c.dt;
+  v.x = v.x + hdt * f.x;
+  v.y = v.y + hdt * f.y;
+  v.z = v.z + hdt * f.z;
+  b.vel[t] = make_float4(v.x, v.y, v.z, 0.f);
+  float px = x.x + c.dt * v.x, py = x.y + c.dt * v.y, pz = x.z + c.dt * v.z;
+  px = px - floorf(px / c.box[0]) * c.box[0];
+  py = py - floorf(py / c.box[1]) * c.box[1];
+  pz = pz - floorf(pz / c.box[2]) * c.box[2];
+  b.pos[t] = make_float4(px, py, pz, x.w);
+}
+
+__global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  if (b.frozen[t] != 0u) {
+    b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  float4 v = b.vel[t], f = b.force[t];
+  float hdt = 0.5f * c.dt;
+  uint32_t s = b.trng[t];
+  float noise[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    s = pcg_soup(s);
+    float u1 = fmaxf(uniform01(s), 1e-7f);
+    s = pcg_soup(s);
+    float u2 = uniform01(s);
+    // Box-Muller: log and cos evaluated in double on the f32 argument, rounded to f32
+    float l = (float)log((double)u1);
+    float cs = (float)cos((double)(6.28318531f * u2));
+    noise[a] = sqrtf(-2.0f * l) * cs;
+  }
+  b.trng[t] = s;
+  float sigma_v = sqrtf(2.0f * c.gamma * c.kT * c.dt);
+  float gdt = c.gamma * c.dt;
+  float vx = v.x + hdt * f.x, vy = v.y + hdt * f.y, vz = v.z + hdt * f.z;
+  vx = vx - gdt * vx + sigma_v * noise[0];
+  vy = vy - gdt * vy + sigma_v * noise[1];
+  vz = vz - gdt * vz + sigma_v * noise[2];
+  b.vel[t] = make_float4(vx, vy, vz, 0.f);
+}
+
+// soup_max_drift_main (verlet.wgsl:115-123): wave max, one atomic per wave.
+__global__ __launch_bounds__(256) void k_max_drift(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bits = 0;
+  if (t < c.n) {
+    float4 x = b.pos[t], x0 = b.pos_at_rebuild[t];
+    float dx = mi_f(x.x - x0.x, c.box[0]), dy = mi_f(x.y - x0.y, c.box[1]), dz = mi_f(x.z - x0.z, c.box[2]);
+    float d2 = dx * dx + dy * dy + dz * dz;
+    bits = __float_as_uint(d2);
+  }
+  bits = wave_max_u(bits);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && bits != 0u) atomicMax(&b.flags[2], bits);
+}
+
+// soup_scan_nonfinite_main (soup/wgsl/health.wgsl:52-68)
+__global__ __launch_bounds__(256) void k_nonfinite(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bp = 0, bv = 0;
+  if (t < c.n) {
+    float4 x = b.pos[t], v = b.vel[t];
+    bp = (uint32_t)nonfinite_bits(x.x) + (uint32_t)nonfinite_bits(x.y) + (uint32_t)nonfinite_bits(x.z);
+    bv = (uint32_t)nonfinite_bits(v.x) + (uint32_t)nonfinite_bits(v.y) + (uint32_t)nonfinite_bits(v.z);
+  }
+  bp = wave_sum_u(bp);
+  bv = wave_sum_u(bv);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (bp) atomicAdd(&b.flags[3], bp);
+    if (bv) atomicAdd(&b.flags[4], bv);
+  }
+}
+
+// soup_relax_step_main (soup/wgsl/relax.wgsl:49-72)
+__global__ __launch_bounds__(256) void k_relax_step(SoupConfig c, DevBuf b, float d) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n || b.frozen[t] != 0u) return;
+  float4 f = b.force[t];
+  float a = fmaxf(fmaxf(fabsf(f.x), fabsf(f.y)), fabsf(f.z));
+  if (!(a > 0.0f) || nonfinite_bits(a)) return;
+  float gx = f.x / a, gy = f.y / a, gz = f.z / a;
+  float l = len3(gx, gy, gz);
+  float4 x = b.pos[t];
+  float px = x.x + (gx / l) * d, py = x.y + (gy / l) * d, pz = x.z + (gz / l) * d;
+  px = px - floorf(px / c.box[0]) * c.box[0];
+  py = py - floorf(py / c.box[1]) * c.box[1];
+  pz = pz - floorf(pz / c.box[2]) * c.box[2];
+  b.pos[t] = make_float4(px, py, pz, x.w);
+}
+
+// ================================================================================================
+// Rebuild: stable counting sort by (cell, particle index) + permutation of all state
+// (replaces clear/count/prefix/fill_main, engine/wgsl/neighbor.wgsl:26-64)
+// ================================================================================================
+__global__ __launch_bounds__(256) void k_cell_count(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  uint32_t cell = cell_of(c, b.pos[t]);
+  b.cell_tmp[t] = cell;
+  b.cell_rank[t] = atomicAdd(&b.cell_count[cell], 1u);
+}
+
+// unordered fill into the cell segment, then rank by particle index inside the segment
+__global__ __launch_bounds__(256) void k_cell_fill(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  b.perm[b.cell_start[b.cell_tmp[t]] + b.cell_rank[t]] = t;  // perm used as scratch here
+}
+
+__global__ __launch_bounds__(256) void k_cell_rank(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  uint32_t cell = b.cell_tmp[t], me = b.id[t];
+  uint32_t s0 = b.cell_start[cell], s1 = b.cell_start[cell + 1], r = 0;
+  for (uint32_t k = s0; k < s1; ++k) r += (b.id[b.perm[k]] < me) ? 1u : 0u;
+  new_slot_of_old[t] = s0 + r;
+}
+
+// gathers every per-particle array into the new slot order (out = the alternate buffers)
+__global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  uint32_t ns = new_slot_of_old[t];
+  float4 x = b.pos[t];
+  o.pos[ns] = x;
+  o.pos_at_rebuild[ns] = x;  // soup_snapshot_positions_main
+  o.vel[ns] = b.vel[t];
+  o.charge[ns] = b.charge[t];
+  o.bs[ns] = b.bs[t];
+  o.clink[ns] = b.clink[t];
+  o.cheld[ns] = b.cheld[t];
+  o.frozen[ns] = b.frozen[t];
+  o.brng[ns] = b.brng[t];
+  o.trng[ns] = b.trng[t];
+  uint32_t id = b.id[t];
+  o.id[ns] = id;
+  o.cell_of[ns] = b.cell_tmp[t];
+  b.inv[id] = ns;
+}
+
+__global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
+  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nh) b.head_slot[h] = b.inv[b.head_id[h]];
+}
+
+// ================================================================================================
+// Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92)
+// Canonical order: cells z-major (dz, dy, dx in -R..R), slots ascending inside a cell. Cells whose
+// box lies entirely beyond listRange (+0.01 sigma slack) of the particle are skipped; they hold no
+// entry, so the list is unchanged.
+// ================================================================================================
+__global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const float4 xi = b.pos[t];
+  const uint32_t cell = b.cell_of[t];
+  const int D0 = (int)c.dims[0], D1 = (int)c.dims[1], D2 = (int)c.dims[2];
+  const int cx = (int)(cell % c.dims[0]), cy = (int)((cell / c.dims[0]) % c.dims[1]), cz = (int)(cell / (c.dims[0] * c.dims[1]));
+  const float w0 = c.box[0] / (float)D0, w1 = c.box[1] / (float)D1, w2 = c.box[2] / (float)D2;
+  // distance from the particle to its own cell walls
+  const float lo0 = xi.x - (float)cx * w0, lo1 = xi.y - (float)cy * w1, lo2 = xi.z - (float)cz * w2;
+  const float hi0 = w0 - lo0, hi1 = w1 - lo1, hi2 = w2 - lo2;
+  const float cull = (c.list_range + 0.01f) * (c.list_range + 0.01f);
+  const float lr = c.list_range;
+  const float lr_hi = (lr + 0.01f) * (lr + 0.01f);
+  const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
+  const int R = (int)c.walk_radius;
+  const uint32_t cap = c.list_cap, n = c.n;
+  uint32_t count = 0, overflow = 0;
+  for (int dz = -R; dz <= R; ++dz) {
+    float gz = dz > 0 ? (float)(dz - 1) * w2 + hi2 : (dz < 0 ? (float)(-dz - 1) * w2 + lo2 : 0.0f);
+    gz = fmaxf(gz, 0.0f);
+    int zz = cz + dz;
+    zz = zz < 0 ? zz + D2 : (zz >= D2 ? zz - D2 : zz);
+    for (int dy = -R; dy <= R; ++dy) {
+      float gy = dy > 0 ? (float)(dy - 1) * w1 + hi1 : (dy < 0 ? (float)(-dy - 1) * w1 + lo1 : 0.0f);
+      gy = fmaxf(gy, 0.0f);
+      float gyz = gy * gy + gz * gz;
+      if (gyz > cull) continue;
+      int yy = cy + dy;
+      yy = yy < 0 ? yy + D1 : (yy >= D1 ? yy - D1 : yy);
+      for (int dx = -R; dx <= R; ++dx) {
+        float gx = dx > 0 ? (float)(dx - 1) * w0 + hi0 : (dx < 0 ? (float)(-dx - 1) * w0 + lo0 : 0.0f);
+        gx = fmaxf(gx, 0.0f);
+        if (gx * gx + gyz > cull) continue;
+        int xx = cx + dx;
+        xx = xx < 0 ? xx + D0 : (xx >= D0 ? xx - D0 : xx);
+        uint32_t nc = (uint32_t)xx + c.dims[0] * ((uint32_t)yy + c.dims[1] * (uint32_t)zz);
+        uint32_t k0 = b.cell_start[nc], k1 = b.cell_start[nc + 1];
+        for (uint32_t k = k0; k < k1; ++k) {
+          if (k == t) continue;
+          float4 xj = b.pos[k];
+          // cheap reject, then the exact test of the reference
+          float ax = xi.x - xj.x, ay = xi.y - xj.y, az = xi.z - xj.z;
+          ax -= c.box[0] * rintf(ax * ib0);
+          ay -= c.box[1] * rintf(ay * ib1);
+          az -= c.box[2] * rintf(az * ib2);
+          if (ax * ax + ay * ay + az * az > lr_hi) continue;
+          if (dist_f(c, xi, xj) < lr) {
+            if (count < cap) {
+              b.vl[(size_t)count * n + t] = k;
+              count++;
+            } else {
+              overflow = 1;
+            }
+          }
+        }
+      }
+    }
+  }
+  b.vcount[t] = count;
+  if (overflow) b.flags[0] = 1u;
+}
+
+// ================================================================================================
+// DH head list: soup_build_es_list_main (electrostatics-long.wgsl:31-56), one wavefront per head,
+// candidates in ascending head order compacted with a ballot so the row keeps the reference order.
+// ================================================================================================
+__global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (h >= nh) return;
+  const float4 xi = b.pos[b.head_slot[h]];
+  const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f);
+  const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
+  const uint32_t cap = c.es_cap;
+  uint32_t count = 0;
+  for (uint32_t base = 0; base < nh; base += WAVE) {
+    uint32_t g = base + lane;
+    bool hit = false;
+    if (g < nh && g != h) {
+      float4 xj = b.pos[b.head_slot[g]];
+      float ax = xi.x - xj.x, ay = xi.y - xj.y, az = xi.z - xj.z;
+      ax -= c.box[0] * rintf(ax * ib0);
+      ay -= c.box[1] * rintf(ay * ib1);
+      az -= c.box[2] * rintf(az * ib2);
+      if (ax * ax + ay * ay + az * az <= lr_hi) hit = dist_f(c, xi, xj) < lr;
+    }
+    uint64_t mask = __ballot(hit);
+    if (hit) {
+      uint32_t p = count + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (p < cap) b.es[(size_t)h * cap + p] = g;
+    }
+    count += (uint32_t)__popcll(mask);
+  }
+  if (lane == 0) {
+    b.es_count[h] = count < cap ? count : cap;
+    if (count > cap) b.flags[1] = 1u;
+  }
+}
+
+// ================================================================================================
+// Forces: soup_force_list_main (verlet.wgsl:132-149) = bondedForce (step.wgsl:152-193) + the list
+// sum of nonbondedSoup (pair.wgsl:168-204), then soup_es_force_far_main (electrostatics-long.wgsl
+// :63-82). Per-kind-pair constants are staged in LDS.
+// ================================================================================================
+struct PairTab {
+  float b2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];    // pairB^2
+  float cut2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];  // wca_cut(pairB)^2
+  float attr[SOUP_MAX_KINDS * SOUP_MAX_KINDS];  // class-table depth
+  float polar2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];// 1 if both beads are polar (acid-soap eligible)
+};
+
+__device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
+  for (uint32_t k = threadIdx.x; k < SOUP_MAX_KINDS * SOUP_MAX_KINDS; k += blockDim.x) {
+    uint32_t ki = k / SOUP_MAX_KINDS, kj = k % SOUP_MAX_KINDS;
+    float bb = c.sigma * (c.radius[ki] + c.radius[kj]) * 0.5f;
+    float cut = c.wca_factor * bb;
+    T.b2[k] = bb * bb;
+    T.cut2[k] = cut * cut;
+    T.attr[k] = c.attr[c.cls[ki]][c.cls[kj]];
+    T.polar2[k] = (c.cls[ki] == 1u && c.cls[kj] == 1u) ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+}
+
+struct PairConst {
+  float bx, by, bz, hx, hy, hz;
+  float eps24, rc_attr, rc2_lo, rc2_hi, attr_k2, attr_a;
+  float acid, es_A, es_kappa, es_Frc, split2, es_rc;
+};
+
+__device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
+  PairConst p;
+  p.bx = c.box[0]; p.by = c.box[1]; p.bz = c.box[2];
+  p.hx = 0.5f * c.box[0]; p.hy = 0.5f * c.box[1]; p.hz = 0.5f * c.box[2];
+  p.eps24 = 24.0f * c.epsilon;
+  p.rc_attr = c.wca_factor * c.b_tt;
+  p.rc2_lo = p.rc_attr * p.rc_attr;
+  p.rc2_hi = (p.rc_attr + c.wc) * (p.rc_attr + c.wc);
+  p.attr_k2 = SOUP_PI_F / c.wc;                        // sin(2x) = sin(k2 (r - rc))
+  p.attr_a = c.epsilon * SOUP_PI_F / (2.0f * c.wc);
+  p.acid = c.acid_soap;
+  p.es_A = c.es_A; p.es_kappa = c.es_kappa; p.es_Frc = c.es_Frc;
+  p.split2 = c.es_split * c.es_split;
+  p.es_rc = c.es_rc;
+  return p;
+}
+
+// Minimum image with relative (not box-absolute) accuracy: for a pair straddling the periodic
+// boundary the box length is taken off the coordinate near L first, which is exact (Sterbenz), so
+// the small displacement carries only its own rounding. Positions lie in [0, L].
+__device__ __forceinline__ float mi_acc(float a, float b, float L, float hL) {
+  float d = a - b;
+  if (d > hL) d = (a - L) - b;
+  else if (d < -hL) d = a - (b - L);
+  return d;
+}
+
+struct Acc {
+  double x, y, z;
+};
+
+// Accumulates the force on i from j (nonbondedSoup). The WCA core, where pair forces reach 10^3
+// while the net force is O(10), is evaluated in FP64; the smooth attraction and DH terms in f32;
+// the sum is FP64.
+__device__ __forceinline__ void pair_force(const PairConst& P, const PairTab& T, float4 xi, float4 xj, float qi, float qj,
+                                           Acc& f) {
+#pragma clang fp contract(fast)
+  float dx = mi_acc(xi.x, xj.x, P.bx, P.hx);
+  float dy = mi_acc(xi.y, xj.y, P.by, P.hy);
+  float dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
+  float r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 < 1e-12f) return;
+  uint32_t k = ((uint32_t)xi.w * SOUP_MAX_KINDS + (uint32_t)xj.w) & (SOUP_MAX_KINDS * SOUP_MAX_KINDS - 1u);
+  double fr = 0.0;  // (force along d) / r
+  if (r2 < T.cut2[k]) {
+    double d2 = (double)dx * dx + (double)dy * dy + (double)dz * dz;
+    double ir2 = 1.0 / d2;
+    double s2 = (double)T.b2[k] * ir2;
+    double s6 = s2 * s2 * s2;
+    fr = (double)P.eps24 * (2.0 * s6 * s6 - s6) * ir2;
+  }
+  float as = T.attr[k];
+  if (P.acid > 0.0f && T.polar2[k] != 0.0f && ((qi != 0.0f) != (qj != 0.0f))) as += P.acid;
+  float qq = qi * qj;
+  bool shell = as > 0.0f && r2 >= P.rc2_lo && r2 <= P.rc2_hi;
+  bool dh = qq != 0.0f && r2 < P.split2;
+  if (shell || dh) {
+    float r = sqrtf(r2), ir = 1.0f / r;
+    float g = 0.0f;
+    if (shell) g -= as * P.attr_a * sinf(P.attr_k2 * (r - P.rc_attr)) * ir;
+    if (dh && r < P.es_rc) {
+      float u = P.es_A * expf(-P.es_kappa * r) * ir;
+      g += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
+    }
+    fr += (double)g;
+  }
+  if (fr != 0.0) {
+    f.x += fr * (double)dx;
+    f.y += fr * (double)dy;
+    f.z += fr * (double)dz;
+  }
+}
+
+__device__ __forceinline__ void bond_term(const SoupConfig& c, const PairConst& P, float4 xi, float4 xp, bool fene, Acc& f) {
+  float dx = mi_acc(xi.x, xp.x, P.bx, P.hx), dy = mi_acc(xi.y, xp.y, P.by, P.hy), dz = mi_acc(xi.z, xp.z, P.bz, P.hz);
+  double r = fmax(sqrt((double)dx * dx + (double)dy * dy + (double)dz * dz), 1e-6);
+  double dv;
+  if (fene) {
+    double x = r / (double)c.r_inf;
+    dv = (double)c.k_fene * r / (1.0 - x * x);
+  } else {
+    dv = (double)c.k_bend * (r - (double)c.r_bend);
+  }
+  double m = dv / r;
+  f.x -= m * dx;
+  f.y -= m * dy;
+  f.z -= m * dz;
+}
+
+// bondedForce via inv[] (partners are particle indices)
+__device__ __forceinline__ void bonded_force(const SoupConfig& c, const PairConst& P, const DevBuf& b, uint32_t t, float4 xi,
+                                             Acc& f) {
+  uint4 s = b.bs[t];
+  uint32_t me = b.id[t];
+  uint32_t sl[3] = {s.x, s.y, s.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t p = sl[k];
+    if (p == SOUP_NONE) continue;
+    uint32_t u = b.inv[p];
+    bond_term(c, P, xi, b.pos[u], true, f);
+    uint4 ms = b.bs[u];
+    uint32_t ml[3] = {ms.x, ms.y, ms.z};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      uint32_t o = ml[q];
+      if (o == SOUP_NONE || o == me) continue;
+      bond_term(c, P, xi, b.pos[b.inv[o]], false, f);
+    }
+  }
+  uint32_t owner = b.clink[t];
+  if (owner != SOUP_NONE) bond_term(c, P, xi, b.pos[b.inv[owner]], true, f);
+}
+
+__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b) {
+  __shared__ PairTab T;
+  load_pair_tab(c, T);
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const PairConst P = make_pair_const(c);
+  float4 xi = b.pos[t];
+  float qi = b.charge[t];
+  Acc f{0.0, 0.0, 0.0};
+  bonded_force(c, P, b, t, xi, f);
+  const uint32_t cnt = b.vcount[t], n = c.n;
+  const uint32_t* row = b.vl + t;
+  uint32_t s = 0;
+  for (; s + 4 <= cnt; s += 4) {
+    uint32_t u0 = row[(size_t)s * n], u1 = row[(size_t)(s + 1) * n], u2 = row[(size_t)(s + 2) * n], u3 = row[(size_t)(s + 3) * n];
+    float4 x0 = b.pos[u0], x1 = b.pos[u1], x2 = b.pos[u2], x3 = b.pos[u3];
+    float q0 = b.charge[u0], q1 = b.charge[u1], q2 = b.charge[u2], q3 = b.charge[u3];
+    pair_force(P, T, xi, x0, qi, q0, f);
+    pair_force(P, T, xi, x1, qi, q1, f);
+    pair_force(P, T, xi, x2, qi, q2, f);
+    pair_force(P, T, xi, x3, qi, q3, f);
+  }
+  for (; s < cnt; ++s) {
+    uint32_t u = row[(size_t)s * n];
+    pair_force(P, T, xi, b.pos[u], qi, b.charge[u], f);
+  }
+  b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
+}
+
+// O(N^2) reference force: soup_force_brute_main (step.wgsl:277-292), test path only
+__global__ __launch_bounds__(256) void k_force_brute(SoupConfig c, DevBuf b) {
+  __shared__ PairTab T;
+  load_pair_tab(c, T);
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const PairConst P = make_pair_const(c);
+  float4 xi = b.pos[t];
+  float qi = b.charge[t];
+  Acc f{0.0, 0.0, 0.0};
+  bonded_force(c, P, b, t, xi, f);
+  for (uint32_t u = 0; u < c.n; ++u) {
+    if (u == t) continue;
+    pair_force(P, T, xi, b.pos[u], qi, b.charge[u], f);
+  }
+  b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
+}
+
+__device__ __forceinline__ void far_pair(const PairConst& P, float4 xi, float4 xj, float qi, float qj, Acc& f) {
+#pragma clang fp contract(fast)
+  float qq = qi * qj;
+  if (qq == 0.0f) return;
+  float dx = mi_acc(xi.x, xj.x, P.bx, P.hx), dy = mi_acc(xi.y, xj.y, P.by, P.hy), dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
+  float r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 < P.split2) return;
+  float r = sqrtf(r2);
+  if (r >= P.es_rc) return;
+  float ir = 1.0f / r;
+  float u = P.es_A * expf(-P.es_kappa * r) * ir;
+  double fr = (double)(qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir);
+  f.x += fr * (double)dx;
+  f.y += fr * (double)dy;
+  f.z += fr * (double)dz;
+}
+
+// far DH from the head list, one wavefront per head
+__global__ __launch_bounds__(256) void k_es_far(SoupConfig c, DevBuf b, uint32_t nh) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (h >= nh) return;
+  const uint32_t t = b.head_slot[h];
+  const float qi = b.charge[t];
+  if (qi == 0.0f) return;
+  const PairConst P = make_pair_const(c);
+  const float4 xi = b.pos[t];
+  const uint32_t cnt = b.es_count[h];
+  const uint32_t* row = b.es + (size_t)h * c.es_cap;
+  Acc f{0.0, 0.0, 0.0};
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    uint32_t u = b.head_slot[row[s]];
+    far_pair(P, xi, b.pos[u], qi, b.charge[u], f);
+  }
+  f.x = wave_sum_d(f.x);
+  f.y = wave_sum_d(f.y);
+  f.z = wave_sum_d(f.z);
+  if (lane == 0) {
+    // the near part was rounded to f32 by the list force; add the far part in FP64 and round once
+    float4 g = b.force[t];
+    b.force[t] = make_float4((float)((double)g.x + f.x), (float)((double)g.y + f.y), (float)((double)g.z + f.z), 0.f);
+  }
+}
+
+// far DH over all pairs: soup_es_force_far_brute_main (electrostatics-long.wgsl:87-103), test only
+__global__ __launch_bounds__(256) void k_es_far_brute(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  float qi = b.charge[t];
+  if (qi == 0.0f) return;
+  const PairConst P = make_pair_const(c);
+  float4 xi = b.pos[t];
+  Acc f{0.0, 0.0, 0.0};
+  for (uint32_t u = 0; u < c.n; ++u) {
+    if (u == t) continue;
+    far_pair(P, xi, b.pos[u], qi, b.charge[u], f);
+  }
+  float4 g = b.force[t];
+  b.force[t] = make_float4((float)((double)g.x + f.x), (float)((double)g.y + f.y), (float)((double)g.z + f.z), 0.f);
+}
+
+// ================================================================================================
+// Bond Monte Carlo (soup/wgsl/bond-*.wgsl).
+//
+// Phase A (k_bond_propose, one thread per slot): bondFormWalkList (bond-dispatch.wgsl:212-242) and
+// the draws of bondFormDecide (:104-112). The candidate walk reads only positions, the list and the
+// particle's own slots for partners j > i, and the draws depend only on those, so this phase is
+// order-independent; it records the accepted proposals.
+// Phase B (k_bond_resolve, one workgroup): the state-changing part of bondFormDecide (desorption
+// valves, claims, roll-backs, centre bookkeeping) applied with exactly the result of running the
+// particles in ascending index order, by deterministic reservations: every pending event reserves
+// the particles it may read or write (atomicMin of its rank); an event whose reservations all hold
+// commits. Events that share no particle commute; an event waits for every lower-ranked event it
+// conflicts with.
+// ================================================================================================
+__device__ __forceinline__ int match_rule(const SoupConfig& c, float ti, float tj) {
+  for (uint32_t r = 0; r < SOUP_BOND_RULES; ++r) {
+    float a = c.kindA[r], bb = c.kindB[r];
+    if ((ti == a && tj == bb) || (ti == bb && tj == a)) return (int)r;
+  }
+  return -1;
+}
+__device__ __forceinline__ uint32_t role_of(const SoupConfig& c, uint32_t r, float kind) {
+  return kind == c.kindA[r] ? (uint32_t)c.roleA[r] : (uint32_t)c.roleB[r];
+}
+
+__global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const float4 xi = b.pos[t];
+  const float ti = xi.w;
+  const uint32_t i = b.id[t];
+  const uint4 own = b.bs[t];
+  const float rcat = wca_cut(c, pair_b(c, ti, c.catalyst_kind));
+  uint32_t catNear = 0, catId = SOUP_NONE, nCand = 0;
+  uint32_t candJ[SOUP_CAND_CAP], candR[SOUP_CAND_CAP];
+  const uint32_t cnt = b.vcount[t], n = c.n;
+  for (uint32_t s = 0; s < cnt; ++s) {
+    uint32_t u = b.vl[(size_t)s * n + t];
+    float4 xj = b.pos[u];
+    float tj = xj.w;
+    if (catNear == 0u && tj == c.catalyst_kind) {
+      if (dist_f(c, xi, xj) < rcat) {
+        catNear = 1u;
+        catId = b.id[u];
+      }
+    }
+    if (nCand < SOUP_CAND_CAP) {
+      uint32_t j = b.id[u];
+      if (j > i) {
+        int r = match_rule(c, ti, tj);
+        if (r >= 0 && own.x != j && own.y != j && own.z != j) {
+          if (dist_f(c, xi, xj) < wca_cut(c, pair_b(c, ti, tj))) {
+            candJ[nCand] = j;
+            candR[nCand] = (uint32_t)r;
+            nCand++;
+          }
+        }
+      }
+    }
+  }
+  uint32_t rng = b.brng[t], nacc = 0;
+  for (uint32_t k = 0; k < nCand; ++k) {
+    uint32_t r = candR[k];
+    if (c.req_cat[r] > 0.5f && catNear == 0u) continue;
+    rng = pcg_soup(rng);
+    if (uniform01(rng) >= c.att_form[r]) continue;
+    rng = pcg_soup(rng);
+    if (uniform01(rng) >= c.acc_form[r]) continue;
+    b.props[(size_t)t * SOUP_CAND_CAP + nacc] = make_uint2(candJ[k], r);
+    nacc++;
+    // a propagation proposal may give i, j and the catalyst a centre link: mark them
+    float tj = b.pos[b.inv[candJ[k]]].w;
+    if (role_of(c, r, ti) == 0u && role_of(c, r, tj) == 0u) {
+      b.flag_cc[i] = 1u;
+      b.flag_cc[candJ[k]] = 1u;
+      if (catId != SOUP_NONE) b.flag_cc[catId] = 1u;
+    }
+  }
+  b.brng[t] = rng;
+  b.nacc[t] = nacc;
+  b.cat_id[t] = catId;
+}
+
+// drops termination proposals that cannot succeed in any order (the carbon has no centre and no
+// propagation proposal can give it one), then flags the particles that carry an event.
+__global__ __launch_bounds__(256) void k_bond_filter(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t i = b.id[t];
+  const float ti = b.pos[t].w;
+  uint32_t nacc = b.nacc[t], keep = 0;
+  for (uint32_t k = 0; k < nacc; ++k) {
+    uint2 p = b.props[(size_t)t * SOUP_CAND_CAP + k];
+    uint32_t j = p.x, r = p.y;
+    uint32_t uj = b.inv[j];
+    float tj = b.pos[uj].w;
+    uint32_t roleI = role_of(c, r, ti), roleJ = role_of(c, r, tj);
+    bool live = true;
+    if (!(roleI == 0u && roleJ == 0u)) {
+      uint32_t carbon = roleI == 1u ? i : j;
+      uint32_t cs = roleI == 1u ? t : uj;
+      live = b.clink[cs] != SOUP_NONE || b.flag_cc[carbon] != 0u;
+    }
+    if (live) b.props[(size_t)t * SOUP_CAND_CAP + keep++] = p;
+  }
+  b.nacc[t] = keep;
+  if (keep > 0u || b.clink[t] != SOUP_NONE || b.flag_cc[i] != 0u) b.ev_flag[i] = 1u;
+}
+
+__global__ __launch_bounds__(256) void k_bond_events(SoupConfig c, DevBuf b) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  if (b.ev_flag[i]) b.ev_id[b.ev_pos[i]] = i;
+}
+
+// ---- phase B helpers: all shared state accessed with device-scope atomics ----------------------
+__device__ __forceinline__ uint32_t ld(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool cas_u(uint32_t* p, uint32_t expected, uint32_t v) {
+  uint32_t e = expected;
+  return __hip_atomic_compare_exchange_strong(p, &e, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct BondCtx {
+  const SoupConfig& c;
+  const DevBuf& b;
+  __device__ uint32_t* slots(uint32_t id) const { return reinterpret_cast<uint32_t*>(&b.bs[b.inv[id]]); }
+  __device__ uint32_t* cl(uint32_t id) const { return &b.clink[b.inv[id]]; }
+  __device__ uint32_t* held(uint32_t id) const { return &b.cheld[b.inv[id]]; }
+  __device__ float4 pos(uint32_t id) const { return b.pos[b.inv[id]]; }
+  __device__ float dist(uint32_t a, uint32_t bb) const { return dist_f(c, pos(a), pos(bb)); }
+  __device__ bool contact_cat(uint32_t a, uint32_t cat) const {
+    return dist(a, cat) < wca_cut(c, pair_b(c, pos(a).w, c.catalyst_kind));
+  }
+  __device__ bool within_reach(uint32_t a, uint32_t owner) const {
+    return dist(a, owner) < wca_cut(c, pair_b(c, pos(a).w, c.catalyst_kind)) + c.wc;
+  }
+};
+
+// tryClaimSlot (bond-valence.wgsl:208-266)
+__device__ int try_claim(const BondCtx& X, uint32_t particle, uint32_t role, uint32_t partner) {
+  uint32_t* sl = X.slots(particle);
+  bool hto = X.c.head_terminal_only > 0.5f;
+  if (role == 0u) {
+    if (cas_u(&sl[0], SOUP_NONE, partner)) {
+      if (hto && ld(&sl[2]) != SOUP_NONE) {
+        st(&sl[0], SOUP_NONE);
+        return -1;
+      }
+      return 0;
+    }
+    if (cas_u(&sl[1], SOUP_NONE, partner)) {
+      if (hto && ld(&sl[2]) != SOUP_NONE) {
+        st(&sl[1], SOUP_NONE);
+        return -1;
+      }
+      return 1;
+    }
+    return -1;
+  } else if (role == 1u) {
+    if (cas_u(&sl[2], SOUP_NONE, partner)) {
+      if (hto && ld(&sl[0]) != SOUP_NONE && ld(&sl[1]) != SOUP_NONE) {
+        st(&sl[2], SOUP_NONE);
+        return -1;
+      }
+      return 2;
+    }
+    return -1;
+  }
+  uint32_t cap = (uint32_t)X.c.head_chain_cap;
+  for (uint32_t k = 0; k < cap; ++k)
+    if (cas_u(&sl[k], SOUP_NONE, partner)) return (int)k;
+  return -1;
+}
+
+__device__ void release_slot(const BondCtx& X, uint32_t particle, int slot) {
+  if (slot >= 0) st(&X.slots(particle)[slot], SOUP_NONE);
+}
+
+// propagateOnCenter (bond-adsorption.wgsl:87-248)
+__device__ bool propagate(const BondCtx& X, uint32_t i, uint32_t j, int slotI, int slotJ, uint32_t catId) {
+  if (slotI == 0 && slotJ == 0) {
+    if (catId == SOUP_NONE) return false;
+    if (!X.contact_cat(i, catId)) return false;
+    if (!cas_u(X.cl(catId), SOUP_NONE, i)) return false;
+    if (!cas_u(X.cl(i), SOUP_NONE, catId)) {
+      cas_u(X.cl(catId), i, SOUP_NONE);
+      return false;
+    }
+    st(X.held(catId), 0u);
+    return true;
+  }
+  uint32_t tipOld, tipNew;
+  if (slotI == 1 && slotJ == 0) {
+    tipOld = i;
+    tipNew = j;
+  } else if (slotI == 0 && slotJ == 1) {
+    tipOld = j;
+    tipNew = i;
+  } else {
+    return false;
+  }
+  uint32_t owner = ld(X.cl(tipOld));
+  bool reclaiming = false;
+  if (owner == SOUP_NONE) {
+    if (catId == SOUP_NONE) return false;
+    owner = catId;
+    reclaiming = true;
+  }
+  if (reclaiming) {
+    if (!X.contact_cat(tipOld, catId)) return false;
+  } else {
+    if (!X.within_reach(tipOld, owner)) return false;
+  }
+  if (!X.contact_cat(tipNew, owner)) return false;
+  if (reclaiming) {
+    if (!cas_u(X.cl(owner), SOUP_NONE, tipOld)) return false;
+    if (!cas_u(X.cl(tipOld), SOUP_NONE, owner)) {
+      cas_u(X.cl(owner), tipOld, SOUP_NONE);
+      return false;
+    }
+  }
+  if (!cas_u(X.cl(owner), tipOld, tipNew)) {
+    if (reclaiming) cas_u(X.cl(tipOld), owner, SOUP_NONE);
+    return false;
+  }
+  if (!cas_u(X.cl(tipNew), SOUP_NONE, owner)) {
+    cas_u(X.cl(owner), tipNew, tipOld);
+    if (reclaiming) {
+      cas_u(X.cl(owner), tipOld, SOUP_NONE);
+      cas_u(X.cl(tipOld), owner, SOUP_NONE);
+    }
+    return false;
+  }
+  cas_u(X.cl(tipOld), owner, SOUP_NONE);
+  st(X.held(owner), 0u);
+  return true;
+}
+
+// terminateOnCenter (bond-adsorption.wgsl:256-268)
+__device__ bool terminate(const BondCtx& X, uint32_t carbon) {
+  uint32_t owner = ld(X.cl(carbon));
+  if (owner == SOUP_NONE) return false;
+  if (!X.within_reach(carbon, owner)) return false;
+  if (!cas_u(X.cl(carbon), owner, SOUP_NONE)) return false;
+  cas_u(X.cl(owner), carbon, SOUP_NONE);
+  st(X.held(owner), 0u);
+  return true;
+}
+
+// the whole per-particle turn of bondFormDecide after the draws
+__device__ void bond_event(const BondCtx& X, uint32_t i) {
+  const SoupConfig& c = X.c;
+  const DevBuf& b = X.b;
+  const uint32_t t = b.inv[i];
+  const float ti = b.pos[t].w;
+  if (ti == c.catalyst_kind) {
+    // desorbTimeout (bond-adsorption.wgsl:304-316)
+    uint32_t tip = ld(X.cl(i));
+    if (tip != SOUP_NONE) {
+      uint32_t inc = (uint32_t)c.bond_interval_f;
+      uint32_t prev = ld(X.held(i));
+      st(X.held(i), prev + inc);
+      if (prev + inc > (uint32_t)c.max_hold_steps) {
+        if (cas_u(X.cl(i), tip, SOUP_NONE)) {
+          cas_u(X.cl(tip), i, SOUP_NONE);
+          st(X.held(i), 0u);
+          atomicAdd(&b.desorb[1], 1u);
+        }
+      }
+    }
+  } else {
+    // desorbStretch (bond-adsorption.wgsl:283-292)
+    uint32_t owner = ld(X.cl(i));
+    if (owner != SOUP_NONE && X.dist(i, owner) >= c.r_inf) {
+      if (cas_u(X.cl(i), owner, SOUP_NONE)) {
+        cas_u(X.cl(owner), i, SOUP_NONE);
+        st(X.held(owner), 0u);
+        atomicAdd(&b.desorb[0], 1u);
+      }
+    }
+  }
+  const uint32_t nacc = b.nacc[t], catId = b.cat_id[t];
+  for (uint32_t k = 0; k < nacc; ++k) {
+    uint2 p = b.props[(size_t)t * SOUP_CAND_CAP + k];
+    uint32_t j = p.x, r = p.y;
+    float tj = X.pos(j).w;
+    uint32_t roleI = role_of(c, r, ti), roleJ = role_of(c, r, tj);
+    int slotI = try_claim(X, i, roleI, j);
+    if (slotI < 0) continue;
+    int slotJ = try_claim(X, j, roleJ, i);
+    if (slotJ < 0) {
+      release_slot(X, i, slotI);
+      continue;
+    }
+    bool ok;
+    if (roleI == 0u && roleJ == 0u) ok = propagate(X, i, j, slotI, slotJ, catId);
+    else if (roleI == 1u) ok = terminate(X, i);
+    else ok = terminate(X, j);
+    if (!ok) {
+      release_slot(X, i, slotI);
+      release_slot(X, j, slotJ);
+      continue;
+    }
+    atomicAdd(&b.events[r * 2u], 1u);
+  }
+}
+
+// every particle index the event of particle i may read or write (superset)
+__device__ uint32_t bond_event_resources(const BondCtx& X, uint32_t i, uint32_t* R) {
+  const DevBuf& b = X.b;
+  const uint32_t t = b.inv[i];
+  uint32_t m = 0;
+  auto add = [&](uint32_t x) {
+    if (x == SOUP_NONE) return;
+    for (uint32_t k = 0; k < m; ++k)
+      if (R[k] == x) return;
+    R[m++] = x;
+  };
+  add(i);
+  add(ld(X.cl(i)));
+  const uint32_t nacc = b.nacc[t];
+  if (nacc > 0u) {
+    add(b.cat_id[t]);
+    for (uint32_t k = 0; k < nacc; ++k) {
+      uint32_t j = b.props[(size_t)t * SOUP_CAND_CAP + k].x;
+      add(j);
+      add(ld(X.cl(j)));
+    }
+  }
+  return m;
+}
+
+__device__ __forceinline__ void phase_barrier() {
+  __threadfence();
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_bond_resolve(SoupConfig c, DevBuf b) {
+  __shared__ uint32_t s_pending;
+  const uint32_t nev = b.ev_pos[c.n];
+  const BondCtx X{c, b};
+  for (uint32_t e = threadIdx.x; e < nev; e += blockDim.x) st(&b.ev_state[e], 0u);
+  phase_barrier();
+  for (uint32_t round = 1;; ++round) {
+    // reserve
+    for (uint32_t e = threadIdx.x; e < nev; e += blockDim.x) {
+      if (ld(&b.ev_state[e]) & 1u) continue;
+      uint32_t* R = b.rset + (size_t)e * SOUP_RSET;
+      uint32_t m = bond_event_resources(X, b.ev_id[e], R + 1);
+      R[0] = m;
+      for (uint32_t k = 0; k < m; ++k) atomicMin(&b.resv[R[1 + k]], e);
+      st(&b.ev_state[e], round << 8);
+    }
+    phase_barrier();
+    // commit: an event whose reservations all hold executes its whole turn
+    for (uint32_t e = threadIdx.x; e < nev; e += blockDim.x) {
+      uint32_t s = ld(&b.ev_state[e]);
+      if ((s & 1u) || (s >> 8) != round) continue;
+      const uint32_t* R = b.rset + (size_t)e * SOUP_RSET;
+      bool mine = true;
+      for (uint32_t k = 0; k < R[0]; ++k)
+        if (ld(&b.resv[R[1 + k]]) != e) {
+          mine = false;
+          break;
+        }
+      if (mine) {
+        bond_event(X, b.ev_id[e]);
+        st(&b.ev_state[e], s | 1u);
+      }
+    }
+    if (threadIdx.x == 0) s_pending = 0;
+    phase_barrier();
+    // release every reservation of this round, count what is left
+    uint32_t pending = 0;
+    for (uint32_t e = threadIdx.x; e < nev; e += blockDim.x) {
+      uint32_t s = ld(&b.ev_state[e]);
+      if ((s >> 8) == round) {
+        const uint32_t* R = b.rset + (size_t)e * SOUP_RSET;
+        for (uint32_t k = 0; k < R[0]; ++k) st(&b.resv[R[1 + k]], SOUP_NONE);
+      }
+      if (!(s & 1u)) pending++;
+    }
+    if (pending) atomicAdd(&s_pending, pending);
+    phase_barrier();
+    uint32_t left = s_pending;
+    __syncthreads();
+    if (left == 0u) break;
+    // the lowest pending event always commits, so nev rounds always suffice
+    if (round > nev + 2u) {
+      if (threadIdx.x == 0) st(&b.flags[6], 1u);
+      break;
+    }
+  }
+}
+
+// bond_break_main (bond-dispatch.wgsl:170-200): each particle breaks only bonds to partners >= i,
+// and releases the partner's slot holding i, which the partner's own pass never reads.
+__global__ __launch_bounds__(256) void k_bond_break(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t i = b.id[t];
+  const float ti = b.pos[t].w;
+  uint32_t* sl = reinterpret_cast<uint32_t*>(&b.bs[t]);
+  uint32_t rng = b.brng[t];
+  for (uint32_t k = 0; k < 3; ++k) {
+    uint32_t p = ld(&sl[k]);
+    if (p == SOUP_NONE || p < i) continue;
+    uint32_t up = b.inv[p];
+    int r = match_rule(c, ti, b.pos[up].w);
+    if (r < 0) continue;
+    rng = pcg_soup(rng);
+    if (uniform01(rng) >= c.att_break[r]) continue;
+    rng = pcg_soup(rng);
+    if (uniform01(rng) >= c.acc_break[r]) continue;
+    st(&sl[k], SOUP_NONE);
+    uint32_t* ps = reinterpret_cast<uint32_t*>(&b.bs[up]);
+    for (uint32_t q = 0; q < 3; ++q)
+      if (cas_u(&ps[q], i, SOUP_NONE)) break;
+    atomicAdd(&b.events[(uint32_t)r * 2u + 1u], 1u);
+  }
+  b.brng[t] = rng;
+}
+
+// ================================================================================================
+// Constant-pH Monte Carlo (soup/src/electrostatics.ts:435-496, acid-soap.ts:170-198) on the device.
+// k_proton_pairs tabulates, for every head-list entry, the FP64 unit-charge screened-Coulomb energy
+// (0 beyond rc) and the acid-soap well (0 beyond its reach) at the current positions, with the
+// reference's double-precision minimum image (Math.round). k_proton_chain then runs the serial
+// chain in one workgroup: per attempt, the site draw, an FP64 reduction of the tabulated terms
+// weighted by the current charges, the Metropolis test with the draw only when dG > 0.
+// ================================================================================================
+__device__ __forceinline__ double js_round(double x) {
+  double r = floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
+}
+__device__ __forceinline__ double pcg_next_u(uint32_t& s) {
+  uint32_t v = s * 747796405u + 2891336453u;
+  s = v;
+  v = ((v >> ((v >> 28) + 4u)) ^ v) * 277803737u;
+  v = (v >> 22) ^ v;
+  return (double)v / 4294967296.0;
+}
+
+__global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, uint32_t nh, double2* tab) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  if (h >= nh) return;
+  const float4 xi = b.pos[b.head_slot[h]];
+  const uint32_t cnt = b.es_count[h];
+  const double bx = c.box[0], by = c.box[1], bz = c.box[2];
+  const double cut2 = c.ph_cutoff * c.ph_cutoff;
+  const double reach = c.as_rc + c.as_wc, reach2 = reach * reach;
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    uint32_t g = b.es[(size_t)h * c.es_cap + s];
+    float4 xj = b.pos[b.head_slot[g]];
+    double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
+    dx = dx - js_round(dx / bx) * bx;
+    dy = dy - js_round(dy / by) * by;
+    dz = dz - js_round(dz / bz) * bz;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    double e = 0.0, w = 0.0;
+    if (r2 < cut2 && c.ph_A != 0.0) {
+      double r = sqrt(r2);
+      if (r > 0.0 && r < c.ph_cutoff) {
+        double u1 = (c.ph_A * exp(-c.ph_kappa * r)) / r;
+        e = u1 - c.ph_shiftU + c.ph_shiftF * (r - c.ph_cutoff);
+      }
+    }
+    if (c.as_scale > 0.0 && r2 < reach2 && r2 > 0.0) {
+      double r = sqrt(r2);
+      if (r < c.as_rc) w = -c.as_scale * c.as_eps;
+      else if (r > c.as_rc + c.as_wc) w = 0.0;
+      else {
+        double cc = cos((3.141592653589793 * (r - c.as_rc)) / (2.0 * c.as_wc));
+        w = -c.as_scale * c.as_eps * cc * cc;
+      }
+    }
+    tab[(size_t)h * c.es_cap + s] = make_double2(e, w);
+  }
+}
+
+#define CHAIN_THREADS 256
+// stats: [0] attempts, [1] accepted, [2] deprotonated, [3] dEsCount; dstats: [0] dEsSum
+__global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, DevBuf b, uint32_t nh, const double2* tab,
+                                                                uint32_t* rng_io, uint32_t* stats, double* dstats) {
+  extern __shared__ float qv[];  // charge of every head
+  __shared__ double s_part[2][2][CHAIN_THREADS / WAVE];
+  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE, nw = CHAIN_THREADS / WAVE;
+  for (uint32_t g = tid; g < nh; g += CHAIN_THREADS) qv[g] = b.charge[b.head_slot[g]];
+  __syncthreads();
+  uint32_t state = *rng_io;
+  const double kT = c.ph_kT, q = c.ph_q;
+  const double dGintr = kT * 2.302585092994046 * (c.ph_pKa - c.ph_pH);
+  uint32_t accepted = 0, dEsCount = 0;
+  double dEsSum = 0.0;
+  for (uint32_t a = 0; a < nh; ++a) {
+    double u = pcg_next_u(state);
+    double hf = floor(u * (double)nh);
+    uint32_t h = hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
+    const uint32_t cnt = b.es_count[h];
+    const uint32_t* row = b.es + (size_t)h * c.es_cap;
+    const double2* trow = tab + (size_t)h * c.es_cap;
+    double se = 0.0, sa = 0.0;
+    for (uint32_t s = tid; s < cnt; s += CHAIN_THREADS) {
+      double2 ew = trow[s];
+      float qj = qv[row[s]];
+      if (qj != 0.0f) se += (q * (double)qj) * ew.x;
+      sa += qj == 0.0f ? ew.y : -ew.y;
+    }
+    se = wave_sum_d(se);
+    sa = wave_sum_d(sa);
+    if (lane == 0) {
+      s_part[a & 1u][0][w] = se;
+      s_part[a & 1u][1][w] = sa;
+    }
+    __syncthreads();
+    double te = 0.0, ta = 0.0;
+    for (uint32_t k = 0; k < nw; ++k) {
+      te += s_part[a & 1u][0][k];
+      ta += s_part[a & 1u][1][k];
+    }
+    double uEs = te + (c.as_scale > 0.0 ? ta : 0.0);
+    bool isCharged = qv[h] != 0.0f;
+    double dG = isCharged ? -(dGintr + uEs) : dGintr + uEs;
+    bool acc = dG <= 0.0;
+    if (!acc) acc = pcg_next_u(state) < exp(-dG / kT);
+    if (acc) {
+      qv[h] = isCharged ? 0.0f : (float)q;  // every thread writes the same value to its own view
+      accepted++;
+      if (!isCharged) {
+        dEsSum += uEs / kT;
+        dEsCount++;
+      }
+    }
+  }
+  __syncthreads();
+  uint32_t dep = 0;
+  for (uint32_t g = tid; g < nh; g += CHAIN_THREADS) {
+    float v = qv[g];
+    b.charge[b.head_slot[g]] = v;
+    dep += v != 0.0f ? 1u : 0u;
+  }
+  dep = wave_sum_u(dep);
+  if (lane == 0) atomicAdd(&stats[2], dep);
+  if (tid == 0) {
+    *rng_io = state;
+    stats[0] = nh;
+    stats[1] = accepted;
+    stats[3] = dEsCount;
+    dstats[0] = dEsSum;
+  }
+}
+
+// ================================================================================================
+// Boundary translation between slot order and particle-index order
+// ================================================================================================
+template <typename T>
+__global__ void k_to_index(const T* src, const uint32_t* id, T* dst, uint32_t n) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[id[t]] = src[t];
+}
+template <typename T>
+__global__ void k_from_index(const T* src, const uint32_t* id, T* dst, uint32_t n) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[id[t]];
+}
+// bond slots: uint4 internal -> 3 u32 per particle
+__global__ void k_bs_to_index(const uint4* bs, const uint32_t* id, uint32_t* dst, uint32_t n) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint4 s = bs[t];
+  uint32_t i = id[t];
+  dst[3u * i] = s.x;
+  dst[3u * i + 1] = s.y;
+  dst[3u * i + 2] = s.z;
+}
+__global__ void k_bs_from_index(const uint32_t* src, const uint32_t* id, uint4* bs, uint32_t n) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t i = id[t];
+  bs[t] = make_uint4(src[3u * i], src[3u * i + 1], src[3u * i + 2], 0u);
+}
+// Verlet list in the reference layout: out[i * cap + s] = particle index
+__global__ void k_vl_to_index(const uint32_t* vl, const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n,
+                              uint32_t cap) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  uint32_t i = id[t], cnt = vcount[t];
+  for (uint32_t s = 0; s < cap; ++s) out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)s * n + t]] : SOUP_NONE;
+}
+__global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* head_id, uint32_t* out,
+                              uint32_t nh, uint32_t cap) {
+  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  uint32_t cnt = es_count[h];
+  for (uint32_t s = 0; s < cap; ++s) out[(size_t)h * cap + s] = s < cnt ? head_id[es[(size_t)h * cap + s]] : SOUP_NONE;
+}
+__global__ void k_vcount_to_index(const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[id[t]] = vcount[t];
+}
+// list occupancy: [0] sum verlet, [1] max verlet, [2] sum es, [3] max es (u64 sums in pairs)
+__global__ __launch_bounds__(256) void k_list_stats(const uint32_t* cnt, uint32_t n, unsigned long long* sum, uint32_t* mx) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t v = t < n ? cnt[t] : 0u;
+  uint32_t m = wave_max_u(v);
+  uint32_t s = wave_sum_u(v);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    atomicAdd(sum, (unsigned long long)s);
+    atomicMax(mx, m);
+  }
+}

@@ -1,0 +1,694 @@
+// soup_lib.hip -- host side of libsoup_hip.so: the C-ABI declared in include/soup_abi.h.
+//
+// One handle = one periodic box on the current HIP device, one stream. The step orchestration
+// mirrors soup/src/soup-integrate.ts:83-175 (per step: kick/drift/wrap, [rebuild every
+// rebuildEvery], list force + far DH, drift max, [bond MC every bondAttemptInterval], kick/thermostat;
+// per 1000-step chunk: the non-finite guard, the Verlet/DH guards and the due constant-pH sweep).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "soup_kernels.inl"
+
+#define SOUP_VERSION "libsoup_hip 0.1 (gfx950)"
+
+static thread_local std::string g_create_error;
+
+struct SoupHandle {
+  SoupConfig c;
+  uint32_t n = 0, ncells = 0, nh = 0;
+  hipStream_t stream = nullptr;
+  DevBuf b{};
+  DevBuf alt{};
+  double2* ptab = nullptr;
+  uint32_t* pstats = nullptr;  // [0..3] chain stats, [4] rng
+  double* pdstats = nullptr;
+  unsigned long long* lsum = nullptr;
+  uint32_t* lmax = nullptr;
+  uint8_t* stage = nullptr;    // device staging for boundary translation
+  size_t stage_bytes = 0;
+  uint64_t global_step = 0, next_sweep_at = UINT64_MAX;
+  uint32_t prng = 0;
+  uint32_t sweeps = 0;
+  SoupSweepResult last_sweep{};
+  std::string err;
+  std::vector<void*> allocs;
+};
+
+static int fail(SoupHandle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+static std::string fmt(const char* f, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof(buf), f, ap);
+  va_end(ap);
+  return buf;
+}
+
+#define HC(h, x)                                                                               \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail((h), SOUP_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <typename T>
+static int dalloc(SoupHandle* h, T** p, size_t count) {
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess)
+    return fail(h, SOUP_E_NOMEM, fmt("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e)));
+  h->allocs.push_back(q);
+  *p = reinterpret_cast<T*>(q);
+  return 0;
+}
+
+#define DA(h, p, cnt)                        \
+  do {                                       \
+    int r_ = dalloc((h), &(p), (size_t)(cnt)); \
+    if (r_) return r_;                       \
+  } while (0)
+
+static void fill(SoupHandle* h, uint32_t* p, uint32_t v, uint32_t count) {
+  if (count) hipLaunchKernelGGL(k_fill_u32, dim3(blocks(count, 256)), dim3(256), 0, h->stream, p, v, count);
+}
+
+// exclusive scan of m u32 -> out[0..m]
+static void scan(SoupHandle* h, const uint32_t* in, uint32_t m, uint32_t* out) {
+  uint32_t nb = blocks(m, SCAN_TILE);
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, m, h->b.scan_blocks);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, h->b.scan_blocks, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, m, h->b.scan_blocks, nb, out);
+}
+
+static bool es_on(const SoupHandle* h) { return h->c.es_enabled && h->nh > 0; }
+
+// ---- phases ------------------------------------------------------------------------------------
+static void ph_kick_drift(SoupHandle* h) {
+  hipLaunchKernelGGL(k_kick_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+}
+static void ph_kick_thermo(SoupHandle* h) {
+  hipLaunchKernelGGL(k_kick_thermo, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+}
+static void ph_drift(SoupHandle* h) {
+  hipLaunchKernelGGL(k_max_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+}
+
+// encodeVerletRebuild (soup/src/soup-grid-verlet.ts:61-73): grid, DH list, Verlet list, snapshot,
+// drift reset -- here with the permutation of the state into the new cell order.
+static void ph_rebuild(SoupHandle* h) {
+  const uint32_t n = h->n;
+  fill(h, h->b.cell_count, 0u, h->ncells);
+  hipLaunchKernelGGL(k_cell_count, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  scan(h, h->b.cell_count, h->ncells, h->b.cell_start);
+  hipLaunchKernelGGL(k_cell_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->b.cell_rank);
+  hipLaunchKernelGGL(k_permute, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->alt, h->b.cell_rank);
+  std::swap(h->b.pos, h->alt.pos);
+  std::swap(h->b.vel, h->alt.vel);
+  std::swap(h->b.charge, h->alt.charge);
+  std::swap(h->b.bs, h->alt.bs);
+  std::swap(h->b.clink, h->alt.clink);
+  std::swap(h->b.cheld, h->alt.cheld);
+  std::swap(h->b.frozen, h->alt.frozen);
+  std::swap(h->b.brng, h->alt.brng);
+  std::swap(h->b.trng, h->alt.trng);
+  std::swap(h->b.id, h->alt.id);
+  if (es_on(h)) {
+    hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+    hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+  }
+  hipLaunchKernelGGL(k_verlet_build, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  fill(h, h->b.flags + 2, 0u, 1);
+}
+
+static void ph_force(SoupHandle* h) {
+  hipLaunchKernelGGL(k_force_list, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  if (es_on(h))
+    hipLaunchKernelGGL(k_es_far, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+}
+
+static void ph_force_brute(SoupHandle* h) {
+  hipLaunchKernelGGL(k_force_brute, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  if (es_on(h)) hipLaunchKernelGGL(k_es_far_brute, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+}
+
+// encodeBondChemistry (soup/src/soup-bond-chemistry.ts:24-40): formation, then breaking
+static void ph_bonds(SoupHandle* h) {
+  const uint32_t n = h->n;
+  fill(h, h->b.flag_cc, 0u, n);
+  fill(h, h->b.ev_flag, 0u, n);
+  hipLaunchKernelGGL(k_bond_propose, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_bond_filter, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  scan(h, h->b.ev_flag, n, h->b.ev_pos);
+  hipLaunchKernelGGL(k_bond_events, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_bond_resolve, dim3(1), dim3(1024), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_bond_break, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+}
+
+static void one_step(SoupHandle* h) {
+  const bool do_bonds = (h->global_step % h->c.bond_interval) == 0;
+  const bool do_rebuild = (h->global_step % h->c.rebuild_every) == 0;
+  ph_kick_drift(h);
+  if (do_rebuild) ph_rebuild(h);
+  ph_force(h);
+  ph_drift(h);
+  if (do_bonds) ph_bonds(h);
+  ph_kick_thermo(h);
+  h->global_step++;
+}
+
+static int sync(SoupHandle* h) {
+  HC(h, hipGetLastError());
+  HC(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// ---- guards --------------------------------------------------------------------------------------
+static int nonfinite_counts(SoupHandle* h, uint32_t out[2]) {
+  fill(h, h->b.flags + 3, 0u, 2);
+  hipLaunchKernelGGL(k_nonfinite, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  HC(h, hipMemcpyAsync(out, h->b.flags + 3, 8, hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+
+// assertStateFinite (soup/src/soup-health.ts:55-63)
+static int assert_finite(SoupHandle* h, bool has_since, uint64_t since) {
+  uint32_t c[2];
+  int r = nonfinite_counts(h, c);
+  if (r) return r;
+  if (c[0] == 0 && c[1] == 0) return 0;
+  std::string window = has_since ? fmt(" (appeared in the step interval %llu..%llu)", (unsigned long long)since,
+                                       (unsigned long long)h->global_step)
+                                 : std::string();
+  return fail(h, SOUP_E_NONFINITE,
+              fmt("non-finite state at step=%llu: non-finite components of positions=%u, velocities=%u of %llu%s -- the "
+                  "computation diverged (Inf/NaN), any further numbers from this run are meaningless",
+                  (unsigned long long)h->global_step, c[0], c[1], (unsigned long long)h->n * 3ull, window.c_str()));
+}
+
+static std::string js_fixed(double x, int d) { return fmt("%.*f", d, x); }
+
+// assertVerletSafety (soup/src/soup-grid-verlet.ts:91-138)
+static int assert_lists(SoupHandle* h) {
+  uint32_t f[8];
+  HC(h, hipMemcpyAsync(f, h->b.flags, 32, hipMemcpyDeviceToHost, h->stream));
+  int r = sync(h);
+  if (r) return r;
+  if (f[6] != 0) return fail(h, SOUP_E_DEVICE, "bond Monte Carlo: the ordered resolution of the accepted proposals did not converge");
+  if (f[0] != 0)
+    return fail(h, SOUP_E_OVERFLOW,
+                fmt("Verlet list: verletList.listCapacity=%u was insufficient -- at least one particle found more "
+                    "candidates than the list holds (data may have been silently dropped)",
+                    h->c.list_cap));
+  if (es_on(h) && f[1] != 0)
+    return fail(h, SOUP_E_OVERFLOW,
+                fmt("long-range electrostatics list: longRangeListCapacity=%u or the headIdx size (%u) was insufficient at "
+                    "rc_es=%s (heads found=%u) -- data may have been silently dropped",
+                    h->c.es_cap, h->c.n_heads, js_fixed(h->c.es_rc, 4).c_str(), h->nh));
+  float d2;
+  memcpy(&d2, &f[2], 4);
+  double drift = std::sqrt(std::fmax(0.0, (double)d2));
+  double bound = (double)h->c.skin / 2.0;
+  if (!std::isfinite(drift))
+    return fail(h, SOUP_E_DRIFT,
+                "Verlet list: the measured drift is not a number (NaN) -- the state is already non-finite, see "
+                "soup/src/soup-health.ts's assertStateFinite");
+  if (drift > bound + 1e-6)
+    return fail(h, SOUP_E_DRIFT,
+                fmt("Verlet list: the measured drift %s exceeds skin/2=%s -- the analytic bound (verletList.basis) did not "
+                    "hold for the real trajectory, the rebuild was not frequent enough",
+                    js_fixed(drift, 4).c_str(), js_fixed(bound, 4).c_str()));
+  return 0;
+}
+
+// ---- constant-pH sweep -------------------------------------------------------------------------
+static int sweep(SoupHandle* h, SoupSweepResult* out) {
+  ph_rebuild(h);  // DH list at the current positions
+  hipLaunchKernelGGL(k_proton_pairs, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh,
+                     h->ptab);
+  fill(h, h->pstats, 0u, 4);
+  HC(h, hipMemcpyAsync(h->pstats + 4, &h->prng, 4, hipMemcpyHostToDevice, h->stream));
+  size_t lds = (size_t)h->nh * sizeof(float);
+  hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, h->ptab,
+                     h->pstats + 4, h->pstats, h->pdstats);
+  uint32_t st[5];
+  double ds;
+  HC(h, hipMemcpyAsync(st, h->pstats, 20, hipMemcpyDeviceToHost, h->stream));
+  HC(h, hipMemcpyAsync(&ds, h->pdstats, 8, hipMemcpyDeviceToHost, h->stream));
+  int r = sync(h);
+  if (r) return r;
+  h->prng = st[4];
+  SoupSweepResult res;
+  res.attempts = st[0];
+  res.accepted = st[1];
+  res.heads = h->nh;
+  res.deprotonated = st[2];
+  res.alpha = h->nh > 0 ? (double)st[2] / (double)h->nh : 0.0;
+  res.dEsMeanKT = st[3] > 0 ? ds / (double)st[3] : 0.0;
+  if (out) *out = res;
+  h->last_sweep = res;
+  h->sweeps++;
+  // F(x) is stale with respect to the new charges (soup-protonation.ts:259-271)
+  ph_rebuild(h);
+  ph_force(h);
+  return sync(h);
+}
+
+template <typename T>
+static void to_index(SoupHandle* h, const T* src) {
+  hipLaunchKernelGGL(k_to_index<T>, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, src, h->b.id,
+                     reinterpret_cast<T*>(h->stage), h->n);
+}
+
+// ================================================================================================
+// C-ABI
+// ================================================================================================
+extern "C" {
+
+const char* soup_version(void) { return SOUP_VERSION; }
+
+const char* soup_last_error(const SoupHandle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+static int validate(const SoupConfig* c) {
+  if (!c) return fail(nullptr, SOUP_E_INVALID, "soup_create: null config");
+  if (c->n == 0) return fail(nullptr, SOUP_E_INVALID, "createSoup: the starting composition is empty");
+  if (c->n_kinds == 0 || c->n_kinds > SOUP_MAX_KINDS)
+    return fail(nullptr, SOUP_E_INVALID, fmt("data/soup.json: %u monomer kinds, but the shader holds at most 8", c->n_kinds));
+  for (int a = 0; a < 3; ++a)
+    if (c->dims[a] < 2 * c->walk_radius + 1)
+      return fail(nullptr, SOUP_E_INVALID,
+                  fmt("neighbour grid: dims=[%u,%u,%u] needs dims>=%u (2*effectiveWalkRadius+1) on all three axes",
+                      c->dims[0], c->dims[1], c->dims[2], 2 * c->walk_radius + 1));
+  if (c->list_cap == 0 || c->rebuild_every == 0 || c->bond_interval == 0)
+    return fail(nullptr, SOUP_E_INVALID, "soup_create: list capacity, rebuildEvery and bondAttemptInterval must be > 0");
+  if (c->es_enabled && c->n_heads > 0 && c->es_cap == 0)
+    return fail(nullptr, SOUP_E_INVALID, "soup_create: long-range list capacity must be > 0");
+  if (c->es_enabled && (size_t)c->n_heads * sizeof(float) > 150u * 1024u)
+    return fail(nullptr, SOUP_E_INVALID,
+                fmt("soup_create: %u titratable heads exceed the constant-pH chain's LDS budget (38400)", c->n_heads));
+  return 0;
+}
+
+int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** out) {
+  if (!out) return fail(nullptr, SOUP_E_INVALID, "soup_create: null output handle");
+  *out = nullptr;
+  int v = validate(cfg);
+  if (v) return v;
+  if (!init || !init->positions || !init->velocities || !init->bond_slots || !init->center_link || !init->center_held ||
+      !init->frozen || !init->bond_rng || !init->thermo_rng || !init->charges)
+    return fail(nullptr, SOUP_E_INVALID, "soup_create: every initial-state array is required");
+  // every index the kernels follow must be in range before anything reaches the device
+  for (uint32_t i = 0; i < cfg->n; ++i) {
+    float w = init->positions[4u * i + 3];
+    if (!(w >= 0.0f) || w != std::floor(w) || w >= (float)cfg->n_kinds)
+      return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: particle %u has kind %g outside the %u monomer kinds", i, w, cfg->n_kinds));
+    for (int k = 0; k < 3; ++k) {
+      uint32_t p = init->bond_slots[3u * i + k];
+      if (p != SOUP_NONE && (p >= cfg->n || p == i))
+        return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: bond slot %d of particle %u names %u (N=%u)", k, i, p, cfg->n));
+    }
+    uint32_t l = init->center_link[i];
+    if (l != SOUP_NONE && (l >= cfg->n || l == i))
+      return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: centerLink of particle %u names %u (N=%u)", i, l, cfg->n));
+  }
+  SoupHandle* h = new SoupHandle();
+  h->c = *cfg;
+  h->n = cfg->n;
+  h->ncells = cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
+  const uint32_t n = h->n;
+  // heads: ascending particle indices of the charged kind; kinds never change, so this is the
+  // compaction soup_build_head_index_main recomputes at every rebuild
+  std::vector<uint32_t> heads;
+  if (cfg->es_enabled && cfg->n_heads > 0) {
+    for (uint32_t i = 0; i < n; ++i)
+      if (init->positions[4u * i + 3] == (float)cfg->charged_kind) heads.push_back(i);
+  }
+  h->nh = (uint32_t)heads.size();
+  int r = 0;
+  auto cleanup = [&](int code) {
+    std::string e = h->err;
+    soup_destroy(h);
+    g_create_error = e;
+    return code;
+  };
+  if ((r = [&]() -> int {
+        HC(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        DevBuf& b = h->b;
+        DevBuf& a = h->alt;
+        DA(h, b.pos, n); DA(h, a.pos, n);
+        DA(h, b.vel, n); DA(h, a.vel, n);
+        DA(h, b.force, n);
+        DA(h, b.pos_at_rebuild, n);
+        DA(h, b.charge, n); DA(h, a.charge, n);
+        DA(h, b.bs, n); DA(h, a.bs, n);
+        DA(h, b.clink, n); DA(h, a.clink, n);
+        DA(h, b.cheld, n); DA(h, a.cheld, n);
+        DA(h, b.frozen, n); DA(h, a.frozen, n);
+        DA(h, b.brng, n); DA(h, a.brng, n);
+        DA(h, b.trng, n); DA(h, a.trng, n);
+        DA(h, b.id, n); DA(h, a.id, n);
+        DA(h, b.inv, n);
+        DA(h, b.cell_of, n);
+        DA(h, b.cell_count, h->ncells);
+        DA(h, b.cell_start, (size_t)h->ncells + 1);
+        DA(h, b.cell_rank, n);
+        DA(h, b.cell_tmp, n);
+        DA(h, b.perm, n);
+        DA(h, b.vl, (size_t)n * cfg->list_cap);
+        DA(h, b.vcount, n);
+        DA(h, b.flags, 8);
+        DA(h, b.events, 2 * SOUP_MAX_RULES);
+        DA(h, b.desorb, 2);
+        DA(h, b.props, (size_t)n * SOUP_CAND_CAP);
+        DA(h, b.nacc, n);
+        DA(h, b.cat_id, n);
+        DA(h, b.flag_cc, n);
+        DA(h, b.ev_flag, n);
+        DA(h, b.ev_pos, (size_t)n + 1);
+        DA(h, b.ev_id, n);
+        DA(h, b.resv, n);
+        DA(h, b.rset, (size_t)n * SOUP_RSET);
+        DA(h, b.ev_state, n);
+        DA(h, b.scan_blocks, SCAN_TILE + 1);
+        uint32_t nh = h->nh > 0 ? h->nh : 1;
+        DA(h, b.head_id, nh);
+        DA(h, b.head_slot, nh);
+        if (h->nh > 0) {
+          DA(h, b.es, (size_t)h->nh * cfg->es_cap);
+          DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
+        } else {
+          DA(h, b.es, 1);
+        }
+        DA(h, b.es_count, nh);
+        DA(h, h->pstats, 8);
+        DA(h, h->pdstats, 2);
+        DA(h, h->lsum, 2);
+        DA(h, h->lmax, 2);
+        h->stage_bytes = std::max<size_t>((size_t)n * cfg->list_cap * 4, (size_t)n * 16);
+        if (h->nh > 0) h->stage_bytes = std::max<size_t>(h->stage_bytes, (size_t)h->nh * cfg->es_cap * 4);
+        DA(h, h->stage, h->stage_bytes);
+        // alternates share the arrays that are rewritten in place
+        a.force = b.force;
+        a.pos_at_rebuild = b.pos_at_rebuild;
+        a.cell_of = b.cell_of;
+        a.inv = b.inv;
+        // initial state: slot = particle index
+        std::vector<uint4> bs(n);
+        for (uint32_t i = 0; i < n; ++i)
+          bs[i] = make_uint4(init->bond_slots[3u * i], init->bond_slots[3u * i + 1], init->bond_slots[3u * i + 2], 0u);
+        std::vector<uint32_t> ids(n);
+        for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+        HC(h, hipMemcpy(b.pos, init->positions, (size_t)n * 16, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.vel, init->velocities, (size_t)n * 16, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.charge, init->charges, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.bs, bs.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.clink, init->center_link, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.cheld, init->center_held, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.frozen, init->frozen, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.brng, init->bond_rng, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.trng, init->thermo_rng, (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemcpy(b.inv, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        HC(h, hipMemset(b.force, 0, (size_t)n * 16));
+        HC(h, hipMemset(b.flags, 0, 32));
+        uint32_t ev[2 * SOUP_MAX_RULES] = {0};
+        if (init->events)
+          for (uint32_t k = 0; k < 2 * cfg->n_rules && k < 2 * SOUP_MAX_RULES; ++k) ev[k] = init->events[k];
+        HC(h, hipMemcpy(b.events, ev, sizeof(ev), hipMemcpyHostToDevice));
+        uint32_t de[2] = {0, 0};
+        if (init->desorb_events) {
+          de[0] = init->desorb_events[0];
+          de[1] = init->desorb_events[1];
+        }
+        HC(h, hipMemcpy(b.desorb, de, 8, hipMemcpyHostToDevice));
+        HC(h, hipMemset(b.resv, 0xFF, (size_t)n * 4));
+        if (h->nh > 0) HC(h, hipMemcpy(b.head_id, heads.data(), (size_t)h->nh * 4, hipMemcpyHostToDevice));
+        size_t lds = (size_t)h->nh * sizeof(float);
+        if (lds > 64 * 1024) HC(h, hipFuncSetAttribute((const void*)k_proton_chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        h->global_step = init->global_step;
+        h->next_sweep_at = init->next_sweep_at;
+        h->prng = init->protonation_rng;
+        // priming pass (soup/src/sim.ts:396-408)
+        ph_rebuild(h);
+        ph_force(h);
+        return sync(h);
+      }()) != 0)
+    return cleanup(r);
+  *out = h;
+  return 0;
+}
+
+int soup_destroy(SoupHandle* h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int soup_global_step(const SoupHandle* h, uint64_t* out) {
+  if (!h || !out) return SOUP_E_INVALID;
+  *out = h->global_step;
+  return 0;
+}
+
+int soup_step(SoupHandle* h, uint32_t nsteps) {
+  if (!h) return SOUP_E_INVALID;
+  uint32_t done = 0;
+  while (done < nsteps) {
+    uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
+    uint64_t chunk_start = h->global_step;
+    for (uint32_t k = 0; k < chunk; ++k) one_step(h);
+    int r = assert_finite(h, true, chunk_start);
+    if (r) return r;
+    r = assert_lists(h);
+    if (r) return r;
+    if (h->c.es_enabled && h->global_step >= h->next_sweep_at) {
+      h->next_sweep_at = h->global_step + h->c.sweep_every;
+      r = sweep(h, nullptr);
+      if (r) return r;
+    }
+    done += chunk;
+  }
+  return 0;
+}
+
+size_t soup_field_bytes(const SoupHandle* h, int field) {
+  if (!h) return 0;
+  const size_t n = h->n;
+  switch (field) {
+    case SOUP_FIELD_POS:
+    case SOUP_FIELD_VEL:
+    case SOUP_FIELD_FORCE: return n * 16;
+    case SOUP_FIELD_BOND_SLOTS: return n * 12;
+    case SOUP_FIELD_CENTER_LINK:
+    case SOUP_FIELD_CENTER_HELD:
+    case SOUP_FIELD_FROZEN:
+    case SOUP_FIELD_CHARGE:
+    case SOUP_FIELD_BOND_RNG:
+    case SOUP_FIELD_THERMO_RNG:
+    case SOUP_FIELD_VERLET_COUNT:
+    case SOUP_FIELD_CELL_OF:
+    case SOUP_FIELD_CELL_IDX: return n * 4;
+    case SOUP_FIELD_EVENTS: return (size_t)h->c.n_rules * 2 * 4;
+    case SOUP_FIELD_DESORB: return 8;
+    case SOUP_FIELD_VERLET_LIST: return n * h->c.list_cap * 4;
+    case SOUP_FIELD_ES_COUNT: return (size_t)h->nh * 4;
+    case SOUP_FIELD_ES_LIST: return (size_t)h->nh * h->c.es_cap * 4;
+    case SOUP_FIELD_CELL_START: return ((size_t)h->ncells + 1) * 4;
+    case SOUP_FIELD_MAX_DRIFT_SQ: return 4;
+    case SOUP_FIELD_HEAD_IDX: return (size_t)h->nh * 4;
+    default: return 0;
+  }
+}
+
+int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
+  if (!h || !dst) return SOUP_E_INVALID;
+  size_t need = soup_field_bytes(h, field);
+  if (need == 0 && field != SOUP_FIELD_ES_COUNT && field != SOUP_FIELD_ES_LIST && field != SOUP_FIELD_HEAD_IDX)
+    return fail(h, SOUP_E_INVALID, fmt("soup_read: unknown field %d", field));
+  if (bytes != need) return fail(h, SOUP_E_INVALID, fmt("soup_read: field %d needs %zu bytes, got %zu", field, need, bytes));
+  if (need == 0) return 0;
+  const void* src = h->stage;
+  const uint32_t n = h->n;
+  switch (field) {
+    case SOUP_FIELD_POS: to_index(h, h->b.pos); break;
+    case SOUP_FIELD_VEL: to_index(h, h->b.vel); break;
+    case SOUP_FIELD_FORCE: to_index(h, h->b.force); break;
+    case SOUP_FIELD_CHARGE: to_index(h, h->b.charge); break;
+    case SOUP_FIELD_CENTER_LINK: to_index(h, h->b.clink); break;
+    case SOUP_FIELD_CENTER_HELD: to_index(h, h->b.cheld); break;
+    case SOUP_FIELD_FROZEN: to_index(h, h->b.frozen); break;
+    case SOUP_FIELD_BOND_RNG: to_index(h, h->b.brng); break;
+    case SOUP_FIELD_THERMO_RNG: to_index(h, h->b.trng); break;
+    case SOUP_FIELD_CELL_OF: to_index(h, h->b.cell_of); break;
+    case SOUP_FIELD_VERLET_COUNT: to_index(h, h->b.vcount); break;
+    case SOUP_FIELD_BOND_SLOTS:
+      hipLaunchKernelGGL(k_bs_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->b.bs, h->b.id,
+                         reinterpret_cast<uint32_t*>(h->stage), n);
+      break;
+    case SOUP_FIELD_VERLET_LIST:
+      hipLaunchKernelGGL(k_vl_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->b.vl, h->b.vcount, h->b.id,
+                         reinterpret_cast<uint32_t*>(h->stage), n, h->c.list_cap);
+      break;
+    case SOUP_FIELD_ES_LIST:
+      hipLaunchKernelGGL(k_es_to_index, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b.es, h->b.es_count,
+                         h->b.head_id, reinterpret_cast<uint32_t*>(h->stage), h->nh, h->c.es_cap);
+      break;
+    case SOUP_FIELD_ES_COUNT: src = h->b.es_count; break;
+    case SOUP_FIELD_HEAD_IDX: src = h->b.head_id; break;
+    case SOUP_FIELD_CELL_IDX: src = h->b.id; break;  // slot order is the cell-sorted order
+    case SOUP_FIELD_CELL_START: src = h->b.cell_start; break;
+    case SOUP_FIELD_EVENTS: src = h->b.events; break;
+    case SOUP_FIELD_DESORB: src = h->b.desorb; break;
+    case SOUP_FIELD_MAX_DRIFT_SQ: src = h->b.flags + 2; break;
+    default: return fail(h, SOUP_E_INVALID, fmt("soup_read: unknown field %d", field));
+  }
+  HC(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+
+int soup_write(SoupHandle* h, int field, const void* srcp, size_t bytes) {
+  if (!h || !srcp) return SOUP_E_INVALID;
+  size_t need = soup_field_bytes(h, field);
+  if (bytes != need || need == 0) return fail(h, SOUP_E_INVALID, fmt("soup_write: field %d needs %zu bytes, got %zu", field, need, bytes));
+  const uint32_t n = h->n;
+  HC(h, hipMemcpyAsync(h->stage, srcp, bytes, hipMemcpyHostToDevice, h->stream));
+  dim3 g(blocks(n, 256)), bl(256);
+  switch (field) {
+    case SOUP_FIELD_POS: hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.pos, n); break;
+    case SOUP_FIELD_VEL: hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.vel, n); break;
+    case SOUP_FIELD_CHARGE: hipLaunchKernelGGL(k_from_index<float>, g, bl, 0, h->stream, (const float*)h->stage, h->b.id, h->b.charge, n); break;
+    case SOUP_FIELD_CENTER_LINK: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.clink, n); break;
+    case SOUP_FIELD_CENTER_HELD: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.cheld, n); break;
+    case SOUP_FIELD_BOND_RNG: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.brng, n); break;
+    case SOUP_FIELD_THERMO_RNG: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.trng, n); break;
+    case SOUP_FIELD_BOND_SLOTS: hipLaunchKernelGGL(k_bs_from_index, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.bs, n); break;
+    default: return fail(h, SOUP_E_INVALID, fmt("soup_write: field %d is not writable", field));
+  }
+  return sync(h);
+}
+
+int soup_forces(SoupHandle* h, int mode) {
+  if (!h) return SOUP_E_INVALID;
+  switch (mode) {
+    case SOUP_FORCES_REBUILD: ph_rebuild(h); ph_force(h); break;
+    case SOUP_FORCES_BRUTE: ph_force_brute(h); break;
+    case SOUP_FORCES_LIST: ph_force(h); break;
+    default: return fail(h, SOUP_E_INVALID, fmt("soup_forces: unknown mode %d", mode));
+  }
+  return sync(h);
+}
+
+int soup_run_phases(SoupHandle* h, uint32_t mask, uint32_t do_bonds_flag) {
+  if (!h) return SOUP_E_INVALID;
+  (void)do_bonds_flag;
+  if (mask & SOUP_PHASE_KICK_DRIFT) ph_kick_drift(h);
+  if (mask & SOUP_PHASE_REBUILD) ph_rebuild(h);
+  if (mask & SOUP_PHASE_FORCE) ph_force(h);
+  if (mask & SOUP_PHASE_DRIFT) ph_drift(h);
+  if (mask & SOUP_PHASE_BONDS) ph_bonds(h);
+  if (mask & SOUP_PHASE_KICK_THERMO) ph_kick_thermo(h);
+  return sync(h);
+}
+
+int soup_protonation_sweep(SoupHandle* h, SoupSweepResult* out) {
+  if (!h) return SOUP_E_INVALID;
+  if (!h->c.es_enabled) {
+    if (out) memset(out, 0, sizeof(*out));
+    return 0;
+  }
+  h->next_sweep_at = h->global_step + h->c.sweep_every;  // maybeProtonationSweep (soup-protonation.ts:54)
+  if (!es_on(h)) {
+    if (out) memset(out, 0, sizeof(*out));
+    return 0;
+  }
+  return sweep(h, out);
+}
+
+int soup_last_sweep(const SoupHandle* h, uint32_t* sweeps, SoupSweepResult* last) {
+  if (!h) return SOUP_E_INVALID;
+  if (sweeps) *sweeps = h->sweeps;
+  if (last) *last = h->last_sweep;
+  return 0;
+}
+
+int soup_protonation_rng(const SoupHandle* h, uint32_t* out) {
+  if (!h || !out) return SOUP_E_INVALID;
+  *out = h->prng;
+  return 0;
+}
+
+int soup_nonfinite(SoupHandle* h, uint32_t out[2]) {
+  if (!h || !out) return SOUP_E_INVALID;
+  return nonfinite_counts(h, out);
+}
+
+int soup_check_lists(SoupHandle* h) {
+  if (!h) return SOUP_E_INVALID;
+  return assert_lists(h);
+}
+
+int soup_list_stats(SoupHandle* h, SoupListStats* out) {
+  if (!h || !out) return SOUP_E_INVALID;
+  HC(h, hipMemsetAsync(h->lsum, 0, 16, h->stream));
+  HC(h, hipMemsetAsync(h->lmax, 0, 8, h->stream));
+  hipLaunchKernelGGL(k_list_stats, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->b.vcount, h->n, h->lsum, h->lmax);
+  if (es_on(h))
+    hipLaunchKernelGGL(k_list_stats, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b.es_count, h->nh, h->lsum + 1,
+                       h->lmax + 1);
+  unsigned long long s[2];
+  uint32_t m[2];
+  HC(h, hipMemcpyAsync(s, h->lsum, 16, hipMemcpyDeviceToHost, h->stream));
+  HC(h, hipMemcpyAsync(m, h->lmax, 8, hipMemcpyDeviceToHost, h->stream));
+  int r = sync(h);
+  if (r) return r;
+  out->verlet_pairs = s[0];
+  out->verlet_mean = h->n ? (double)s[0] / h->n : 0.0;
+  out->verlet_max = m[0];
+  out->verlet_cap = h->c.list_cap;
+  out->es_pairs = es_on(h) ? s[1] : 0;
+  out->es_mean = es_on(h) && h->nh ? (double)s[1] / h->nh : 0.0;
+  out->es_max = es_on(h) ? m[1] : 0;
+  out->es_cap = h->c.es_cap;
+  return 0;
+}
+
+int soup_relax(SoupHandle* h, uint32_t iterations, float d0) {
+  if (!h) return SOUP_E_INVALID;
+  for (uint32_t k = 0; k < iterations; ++k) {
+    float d = (float)((double)d0 * (1.0 - (double)k / (double)iterations));
+    ph_rebuild(h);
+    ph_force(h);
+    hipLaunchKernelGGL(k_relax_step, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b, d);
+    if ((k + 1) % 25 == 0) {
+      int r = sync(h);
+      if (r) return r;
+    }
+  }
+  ph_rebuild(h);
+  ph_force(h);
+  int r = sync(h);
+  if (r) return r;
+  r = assert_finite(h, false, 0);
+  if (r) return r;
+  return assert_lists(h);
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+"""Builds libsoup_hip.so in-tree for gfx950 with hipcc (no JIT cache, no setup.py)."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(PKG_ROOT, "csrc")
+LIB_DIR = os.path.join(PKG_ROOT, "lib")
+LIB = os.path.join(LIB_DIR, "libsoup_hip.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # exact f32 op order in the parity kernels; the force kernels opt back in locally
+    "-ffp-contract=off",
+    # no packed-FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build_library(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    if not os.path.exists(hipcc):
+        hipcc = "hipcc"
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(PKG_ROOT, "..", "include"), "-o", tmp, os.path.join(CSRC, "soup_lib.hip")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stderr[-6000:])
+    if verbose and res.stderr:
+        print(res.stderr[-2000:])
+    os.replace(tmp, LIB)
+    return LIB

@@ -1,0 +1,166 @@
+"""GPU parity: the HIP kernels against the oracle from the same state (through the C-ABI).
+
+Bit-exact: cell assignment and order, Verlet and DH lists, integer RNG streams, integrator output
+given the same force, bond-MC outcomes, constant-pH decisions. Forces: per particle within 1e-5 of
+the FP64 recomputation, relative to max(|F64_i|, median_j |F64_j|).
+"""
+import numpy as np
+import pytest
+
+import protocell_soup as ps
+from oracle import oracle as orc
+from soup_fixtures import BROTH, amphiphile_opts, charged, force_rel_err, oracle_from_system
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def broth():
+    R = ps.resolve(BROTH)
+    s = ps.createSoup(BROTH)
+    yield R, s
+    s.dispose()
+
+
+def test_priming_lists_match_oracle(broth):
+    R, s = broth
+    o = orc.OracleState(R.cfg, R.init, R.charges0)
+    o.rebuild()
+    assert np.array_equal(s.readDEBUG("CELL_OF"), o.cell_of)
+    assert np.array_equal(s.readDEBUG("CELL_START"), o.cell_start)
+    assert np.array_equal(s.readDEBUG("CELL_IDX"), o.cell_idx)
+    assert np.array_equal(s.verletCount(), o.vcount)
+    assert np.array_equal(s.verletList(), o.verlet_rows())
+
+
+def test_forces_match_fp64(broth):
+    R, s = broth
+    o = oracle_from_system(s, R)
+    o.rebuild()
+    f = s.forces().reshape(-1, 4)[:, :3]
+    f64 = o.force_f64(use_list=False)
+    assert force_rel_err(f, f64) < FORCE_TOL
+    fb = s.forcesBruteForce().reshape(-1, 4)[:, :3]
+    assert force_rel_err(fb, f64) < FORCE_TOL
+
+
+def test_integrator_bit_exact_given_force(broth):
+    R, s = broth
+    s.forces()
+    o = oracle_from_system(s, R)
+    o.force[:] = s.forcesNoRebuildDEBUG()
+    s.runPhasesDEBUG("KICK_DRIFT")
+    o.kick_drift()
+    assert np.array_equal(s.particles(), o.pos)
+    assert np.array_equal(s.velocities(), o.vel)
+    s.runPhasesDEBUG("KICK_THERMO")
+    o.kick_thermo()
+    assert np.array_equal(s.rngState()["thermo"], o.thermo_rng)
+    assert np.array_equal(s.velocities(), o.vel)
+
+
+def test_steps_keep_rng_streams(broth):
+    R, s = broth
+    o = oracle_from_system(s, R)
+    s.step(40)
+    for _ in range(40):
+        o.one_step()
+    assert np.array_equal(s.rngState()["thermo"], o.thermo_rng)
+    assert np.array_equal(s.rngState()["bond"], o.bond_rng)
+    # the trajectories separate from ulp-level force differences (chaos), slowly at first
+    d = np.abs(s.particles() - o.pos).reshape(-1, 4)[:, :3]
+    d = np.minimum(d, 16.0 - d)
+    assert np.median(d) < 1e-3
+
+
+def _bond_parity(s, R):
+    s.forces()
+    o = oracle_from_system(s, R)
+    o.rebuild()
+    assert np.array_equal(s.verletList(), o.verlet_rows())
+    s.runPhasesDEBUG("BONDS")
+    o.bond_form()
+    o.bond_break()
+    assert np.array_equal(s.bondSlots(), o.bond_slots)
+    assert np.array_equal(s.centerLinks(), o.center_link)
+    assert np.array_equal(s.centerHeldSteps(), o.center_held)
+    assert np.array_equal(s.rngState()["bond"], o.bond_rng)
+    assert np.array_equal(s.readDEBUG("EVENTS"), o.events[: 2 * R.cfg.n_rules])
+    assert np.array_equal(s.readDEBUG("DESORB"), o.desorb)
+
+
+def test_relax_then_step_dense():
+    opts = dict(BROTH, seed=5, start={"C": 600, "O": 100, "H": 500, "M": 20, "W": 2000})
+    R = ps.resolve(opts)
+    s = ps.createSoup(opts)
+    try:
+        o = orc.OracleState(R.cfg, R.init, R.charges0)
+        # first relax iteration from the same lattice: displacement cap 0.1 along F/|F|
+        s.relaxColdStart({"iterations": 1, "maxDisplacementSigma": 0.1})
+        assert o.relax(1, 0.1) == 0
+        d = np.abs(s.particles() - o.pos).reshape(-1, 4)[:, :3]
+        assert np.minimum(d, 16.0 - d).max() < 1e-4
+        s.dispose()
+        s = ps.createSoup(opts)
+        res = s.relaxColdStart()
+        assert res["nonFiniteAfter"] == 0 and res["maxForceAfter"] < 100.0 < res["maxForceBefore"]
+        s.step(500)
+        assert s.nonFiniteCount() == {"pos": 0, "vel": 0}
+    finally:
+        s.dispose()
+
+
+def test_bond_mc_bit_exact():
+    opts = dict(BROTH, seed=3, start={"C": 900, "O": 300, "H": 300, "M": 60, "W": 0})
+    R = ps.resolve(opts)
+    s = ps.createSoup(opts)
+    try:
+        for _ in range(6):
+            _bond_parity(s, R)
+            s.step(200)
+        ev = s.events()
+        assert ev["cc_bond"] > 0
+    finally:
+        s.dispose()
+
+
+def test_charged_lists_far_force_and_sweep():
+    opts = charged()
+    R = ps.resolve(opts)
+    s = ps.createSoup(opts)
+    try:
+        s.step(100)
+        s.forces()
+        o = oracle_from_system(s, R)
+        o.rebuild()
+        assert np.array_equal(s.readDEBUG("ES_COUNT"), o.es_count[: o.s.nh])
+        es = s.readDEBUG("ES_LIST").reshape(o.s.nh, -1)
+        assert np.array_equal(es, o.es_list.reshape(-1, R.cfg.es_cap)[: o.s.nh])
+        f = s.forcesNoRebuildDEBUG().reshape(-1, 4)[:, :3]
+        assert force_rel_err(f, o.force_f64(use_list=False)) < FORCE_TOL
+        for _ in range(3):
+            o = oracle_from_system(s, R)
+            r_gpu = s.protonationSweepDEBUG()
+            r_ref = o.protonation_sweep()
+            assert np.array_equal(s.charges(), o.charge)
+            assert s.protonationRngState() == o.s.prng
+            assert r_gpu["accepted"] == r_ref["accepted"] and r_gpu["deprotonated"] == r_ref["deprotonated"]
+            s.step(50)
+    finally:
+        s.dispose()
+
+
+def test_amphiphile_soup_runs():
+    opts = amphiphile_opts(16.0, 200, 4, {"W": 2277}, seed=1)
+    s = ps.createSoup(opts)
+    try:
+        s.relaxColdStart()
+        inv0 = s.invariants()
+        s.step(1000)
+        inv1 = s.invariants()
+        assert inv0["monomers"] == inv1["monomers"]
+        assert s.nonFiniteCount() == {"pos": 0, "vel": 0}
+    finally:
+        s.dispose()
