@@ -19,6 +19,20 @@
 
 static thread_local std::string g_create_error;
 
+// Per-phase device timing with HIP events on the handle's stream (soup_profile): every bracketed
+// launch gets an event pair from a pool; pairs are resolved at the next synchronisation.
+struct Prof {
+  bool on = false;
+  double ms[SOUP_PROF_PHASES] = {0};
+  uint64_t count[SOUP_PROF_PHASES] = {0};
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+};
+
 struct SoupHandle {
   SoupConfig c;
   uint32_t n = 0, ncells = 0, nh = 0;
@@ -38,7 +52,44 @@ struct SoupHandle {
   SoupSweepResult last_sweep{};
   std::string err;
   std::vector<void*> allocs;
+  Prof prof;
 };
+
+static hipEvent_t prof_event(SoupHandle* h) {
+  if (!h->prof.pool.empty()) {
+    hipEvent_t e = h->prof.pool.back();
+    h->prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+// brackets the launches issued between prof_begin and prof_end
+static hipEvent_t prof_begin(SoupHandle* h) {
+  if (!h->prof.on) return nullptr;
+  hipEvent_t a = prof_event(h);
+  (void)hipEventRecord(a, h->stream);
+  return a;
+}
+static void prof_end(SoupHandle* h, int phase, hipEvent_t a) {
+  if (!h->prof.on || !a) return;
+  hipEvent_t b = prof_event(h);
+  (void)hipEventRecord(b, h->stream);
+  h->prof.pending.push_back({phase, a, b});
+}
+static void prof_resolve(SoupHandle* h) {
+  for (auto& p : h->prof.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      h->prof.ms[p.phase] += ms;
+      h->prof.count[p.phase] += 1;
+    }
+    h->prof.pool.push_back(p.a);
+    h->prof.pool.push_back(p.b);
+  }
+  h->prof.pending.clear();
+}
 
 static int fail(SoupHandle* h, int code, const std::string& msg) {
   if (h) h->err = msg;
@@ -99,19 +150,26 @@ static bool es_on(const SoupHandle* h) { return h->c.es_enabled && h->nh > 0; }
 
 // ---- phases ------------------------------------------------------------------------------------
 static void ph_kick_drift(SoupHandle* h) {
+  hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_KICK_DRIFT, e);
 }
 static void ph_kick_thermo(SoupHandle* h) {
+  hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_kick_thermo, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_KICK_THERMO, e);
 }
 static void ph_drift(SoupHandle* h) {
+  hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_max_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_DRIFT, e);
 }
 
 // encodeVerletRebuild (soup/src/soup-grid-verlet.ts:61-73): grid, DH list, Verlet list, snapshot,
 // drift reset -- here with the permutation of the state into the new cell order.
 static void ph_rebuild(SoupHandle* h) {
   const uint32_t n = h->n;
+  hipEvent_t e0 = prof_begin(h);
   fill(h, h->b.cell_count, 0u, h->ncells);
   hipLaunchKernelGGL(k_cell_count, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   scan(h, h->b.cell_count, h->ncells, h->b.cell_start);
@@ -128,18 +186,28 @@ static void ph_rebuild(SoupHandle* h) {
   std::swap(h->b.brng, h->alt.brng);
   std::swap(h->b.trng, h->alt.trng);
   std::swap(h->b.id, h->alt.id);
+  prof_end(h, SOUP_PROF_SORT, e0);
   if (es_on(h)) {
+    hipEvent_t e1 = prof_begin(h);
     hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
     hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    prof_end(h, SOUP_PROF_ES_BUILD, e1);
   }
+  hipEvent_t e2 = prof_begin(h);
   hipLaunchKernelGGL(k_verlet_build, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_VERLET_BUILD, e2);
   fill(h, h->b.flags + 2, 0u, 1);
 }
 
 static void ph_force(SoupHandle* h) {
+  hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_force_list, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
-  if (es_on(h))
+  prof_end(h, SOUP_PROF_FORCE, e);
+  if (es_on(h)) {
+    hipEvent_t e1 = prof_begin(h);
     hipLaunchKernelGGL(k_es_far, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    prof_end(h, SOUP_PROF_ES_FAR, e1);
+  }
 }
 
 static void ph_force_brute(SoupHandle* h) {
@@ -150,6 +218,7 @@ static void ph_force_brute(SoupHandle* h) {
 // encodeBondChemistry (soup/src/soup-bond-chemistry.ts:24-40): formation, then breaking
 static void ph_bonds(SoupHandle* h) {
   const uint32_t n = h->n;
+  hipEvent_t e = prof_begin(h);
   fill(h, h->b.flag_cc, 0u, n);
   fill(h, h->b.ev_flag, 0u, n);
   hipLaunchKernelGGL(k_bond_propose, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -158,6 +227,7 @@ static void ph_bonds(SoupHandle* h) {
   hipLaunchKernelGGL(k_bond_events, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   hipLaunchKernelGGL(k_bond_resolve, dim3(1), dim3(1024), 0, h->stream, h->c, h->b);
   hipLaunchKernelGGL(k_bond_break, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_BONDS, e);
 }
 
 static void one_step(SoupHandle* h) {
@@ -175,6 +245,7 @@ static void one_step(SoupHandle* h) {
 static int sync(SoupHandle* h) {
   HC(h, hipGetLastError());
   HC(h, hipStreamSynchronize(h->stream));
+  if (!h->prof.pending.empty()) prof_resolve(h);
   return 0;
 }
 
@@ -244,8 +315,10 @@ static int sweep(SoupHandle* h, SoupSweepResult* out) {
   fill(h, h->pstats, 0u, 4);
   HC(h, hipMemcpyAsync(h->pstats + 4, &h->prng, 4, hipMemcpyHostToDevice, h->stream));
   size_t lds = (size_t)h->nh * sizeof(float);
+  hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, h->ptab,
                      h->pstats + 4, h->pstats, h->pdstats);
+  prof_end(h, SOUP_PROF_PROTON, e);
   uint32_t st[5];
   double ds;
   HC(h, hipMemcpyAsync(st, h->pstats, 20, hipMemcpyDeviceToHost, h->stream));
@@ -453,9 +526,34 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
   return 0;
 }
 
+int soup_profile(SoupHandle* h, int enable) {
+  if (!h) return SOUP_E_INVALID;
+  int r = sync(h);
+  if (r) return r;
+  h->prof.on = enable != 0;
+  for (int k = 0; k < SOUP_PROF_PHASES; ++k) {
+    h->prof.ms[k] = 0.0;
+    h->prof.count[k] = 0;
+  }
+  return 0;
+}
+
+int soup_profile_read(SoupHandle* h, double* ms, uint64_t* count) {
+  if (!h || !ms || !count) return SOUP_E_INVALID;
+  int r = sync(h);
+  if (r) return r;
+  for (int k = 0; k < SOUP_PROF_PHASES; ++k) {
+    ms[k] = h->prof.ms[k];
+    count[k] = h->prof.count[k];
+  }
+  return 0;
+}
+
 int soup_destroy(SoupHandle* h) {
   if (!h) return 0;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->prof.pending.clear();
+  for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -174,6 +174,16 @@ def amphiphile_soup(box: Sequence[float], n_amph: int, tail_len: int, extra_coun
     pos = np.empty((n, 4), dtype=np.float32)
     for a, ia in enumerate((ix, iy, iz)):
         pos[:, a] = ((ia + 0.5) * spacing[a] + (d[:, a] * 2 - 1) * jf * spacing[a]).astype(np.float32)
+    # chain beads sit on a straight rod at a bond length below the FENE divergence radius, starting
+    # at the segment's first site, with the jitter kept perpendicular to the rod
+    bond = min(float(spacing[0]), 1.0)
+    n_chain = n_amph * per
+    b = np.arange(n_chain) % per
+    x0 = ix[:n_chain] - b
+    pos[:n_chain, 0] = ((x0 + 0.5) * spacing[0] + b * bond).astype(np.float32)
+    for a, ia in ((1, iy), (2, iz)):
+        pos[:n_chain, a] = ((ia[:n_chain] + 0.5) * spacing[a] + (d[:n_chain, a] * 2 - 1) * 0.5 * jf * spacing[a]).astype(np.float32)
+    pos[:, :3] = np.mod(pos[:, :3], np.array(box, dtype=np.float32))
     pos[:, 3] = kinds
     s = math.sqrt(kT)
     vel = np.zeros((n, 4), dtype=np.float32)

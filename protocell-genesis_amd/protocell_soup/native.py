@@ -79,8 +79,10 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_version",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_profile", "soup_profile_read", "soup_version",
 )
+PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_far", "drift", "bonds", "kick_thermo",
+               "proton")
 
 _lib: Optional[C.CDLL] = None
 
@@ -112,6 +114,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_check_lists": (C.c_int, [H]),
         "soup_list_stats": (C.c_int, [H, C.POINTER(SoupListStats)]),
         "soup_relax": (C.c_int, [H, C.c_uint32, C.c_float]),
+        "soup_profile": (C.c_int, [H, C.c_int]),
+        "soup_profile_read": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "soup_version": (C.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -409,6 +409,16 @@ class SoupSystem:
         return {"iterations": iterations, "maxDisplacementStart": d0, "displacementBound": d0 * (iterations + 1) / 2,
                 "maxForceBefore": before[0], "maxForceAfter": after[0], "nonFiniteBefore": before[1], "nonFiniteAfter": after[1]}
 
+    # -- profiling -----------------------------------------------------------------------------------
+    def profile(self, enable: bool) -> None:
+        self._call(self._lib.soup_profile, C.c_int(1 if enable else 0))
+
+    def profileRead(self) -> Dict[str, Dict[str, float]]:
+        ms = (C.c_double * 16)()
+        cnt = (C.c_uint64 * 16)()
+        self._call(self._lib.soup_profile_read, ms, cnt)
+        return {name: {"ms": float(ms[k]), "count": int(cnt[k])} for k, name in enumerate(native.PROF_PHASES)}
+
     # -- test hooks --------------------------------------------------------------------------------
     def runPhasesDEBUG(self, *phases: str) -> None:
         mask = 0
