@@ -76,7 +76,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-sample", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale reference) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n, "dispatches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the soup box runs on one MI355X; "
               "multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
