@@ -13,8 +13,13 @@
 // cell-sorted arrays), in the canonical order of soup/wgsl/verlet.wgsl:66-88 (z-major cell walk,
 // ascending particle index inside each cell).
 //
-// The DH list is row-major, es[h * es_cap + s], over HEAD indices (position of the head among the
-// charged-kind particles in ascending particle-index order, electrostatics-long.wgsl:15-29).
+// The DH list is row-major, es[h * es_cap + s]: row h is the h-th charged-kind particle in ascending
+// particle-index order (electrostatics-long.wgsl:15-29); its entries are the SLOTS of the neighbour
+// heads, kept in ascending head order (the reference's order).
+//
+// Launch geometry: workgroups are dispatched round-robin over the 8 XCDs, so the gather-heavy kernels
+// remap blockIdx so that each XCD owns one contiguous run of slots (a spatial slab) and its 4 MB L2
+// holds that slab's neighbourhood instead of the whole box.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,6 +31,7 @@
 #define SOUP_BOND_RULES 2u // soup/wgsl/bond-common.wgsl:41
 #define SOUP_PI_F 3.14159265f
 #define SOUP_RSET 16u      // resources per bond event (<= 15 used)
+
 
 struct DevBuf {
   float4* pos;
@@ -53,6 +59,10 @@ struct DevBuf {
   uint32_t* vcount;
   uint32_t* head_id;
   uint32_t* head_slot;
+  uint32_t* head_of_slot; // by slot: head index, NONE for other particles
+  uint32_t* head_flag;    // by slot: 1 for a head (scan input)
+  uint32_t* head_pos;     // exclusive scan of head_flag (n + 1)
+  uint32_t* head_rows;    // heads in slot (spatial) order
   uint32_t* es;
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,
@@ -112,6 +122,13 @@ __device__ __forceinline__ uint32_t cell_of(const SoupConfig& c, float4 p) {
   uint32_t cy = cell_axis(p.y, c.box[1], c.dims[1]);
   uint32_t cz = cell_axis(p.z, c.box[2], c.dims[2]);
   return cx + c.dims[0] * (cy + c.dims[1] * cz);
+}
+
+#define SOUP_XCDS 8u
+// logical block of an XCD-aware launch (grid = 8 * per blocks)
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t per = gridDim.x / SOUP_XCDS;
+  return (blockIdx.x % SOUP_XCDS) * per + blockIdx.x / SOUP_XCDS;
 }
 
 __device__ __forceinline__ bool nonfinite_bits(float x) { return ((__float_as_uint(x) >> 23) & 0xFFu) == 0xFFu; }

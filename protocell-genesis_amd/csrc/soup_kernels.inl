@@ -265,76 +265,115 @@ __global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf 
 
 __global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
   uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h < nh) b.head_slot[h] = b.inv[b.head_id[h]];
+  if (h >= nh) return;
+  uint32_t u = b.inv[b.head_id[h]];
+  b.head_slot[h] = u;
+  b.head_of_slot[u] = h;
+  b.head_flag[u] = 1u;
+}
+
+// heads in slot order: the far-force and pair-table kernels walk rows in this spatial order
+__global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < c.n && b.head_flag[t]) b.head_rows[b.head_pos[t]] = b.head_of_slot[t];
 }
 
 // ================================================================================================
-// Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92)
-// Canonical order: cells z-major (dz, dy, dx in -R..R), slots ascending inside a cell. Cells whose
-// box lies entirely beyond listRange (+0.01 sigma slack) of the particle are skipped; they hold no
-// entry, so the list is unchanged.
+// Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92), one wavefront per particle.
+// Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Slots
+// are cell-major with x fastest, so the kept cells of one (dz, dy) row are one contiguous slot range
+// (two when the row wraps in x), visited in canonical order; the 64 lanes sweep that range, and a
+// ballot compacts the hits in order. Rows and cells lying entirely beyond listRange + 0.01 sigma of
+// the particle are skipped: they hold no entry, so the list is unchanged.
 // ================================================================================================
 __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t t = xcd_block() * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   if (t >= c.n) return;
   const float4 xi = b.pos[t];
   const uint32_t cell = b.cell_of[t];
   const int D0 = (int)c.dims[0], D1 = (int)c.dims[1], D2 = (int)c.dims[2];
   const int cx = (int)(cell % c.dims[0]), cy = (int)((cell / c.dims[0]) % c.dims[1]), cz = (int)(cell / (c.dims[0] * c.dims[1]));
   const float w0 = c.box[0] / (float)D0, w1 = c.box[1] / (float)D1, w2 = c.box[2] / (float)D2;
-  // distance from the particle to its own cell walls
   const float lo0 = xi.x - (float)cx * w0, lo1 = xi.y - (float)cy * w1, lo2 = xi.z - (float)cz * w2;
   const float hi0 = w0 - lo0, hi1 = w1 - lo1, hi2 = w2 - lo2;
-  const float cull = (c.list_range + 0.01f) * (c.list_range + 0.01f);
   const float lr = c.list_range;
-  const float lr_hi = (lr + 0.01f) * (lr + 0.01f);
-  const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
+  const float cull = (lr + 0.01f) * (lr + 0.01f);
+  const float sure = (lr - 0.01f) * (lr - 0.01f);  // inside this the exact f32 test cannot fail
   const int R = (int)c.walk_radius;
   const uint32_t cap = c.list_cap, n = c.n;
-  uint32_t count = 0, overflow = 0;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t count = 0;
   for (int dz = -R; dz <= R; ++dz) {
-    float gz = dz > 0 ? (float)(dz - 1) * w2 + hi2 : (dz < 0 ? (float)(-dz - 1) * w2 + lo2 : 0.0f);
-    gz = fmaxf(gz, 0.0f);
+    float gz = fmaxf(dz > 0 ? (float)(dz - 1) * w2 + hi2 : (dz < 0 ? (float)(-dz - 1) * w2 + lo2 : 0.0f), 0.0f);
     int zz = cz + dz;
-    zz = zz < 0 ? zz + D2 : (zz >= D2 ? zz - D2 : zz);
+    float sz = 0.0f;
+    if (zz < 0) { zz += D2; sz = -c.box[2]; } else if (zz >= D2) { zz -= D2; sz = c.box[2]; }
     for (int dy = -R; dy <= R; ++dy) {
-      float gy = dy > 0 ? (float)(dy - 1) * w1 + hi1 : (dy < 0 ? (float)(-dy - 1) * w1 + lo1 : 0.0f);
-      gy = fmaxf(gy, 0.0f);
+      float gy = fmaxf(dy > 0 ? (float)(dy - 1) * w1 + hi1 : (dy < 0 ? (float)(-dy - 1) * w1 + lo1 : 0.0f), 0.0f);
       float gyz = gy * gy + gz * gz;
       if (gyz > cull) continue;
       int yy = cy + dy;
-      yy = yy < 0 ? yy + D1 : (yy >= D1 ? yy - D1 : yy);
+      float sy = 0.0f;
+      if (yy < 0) { yy += D1; sy = -c.box[1]; } else if (yy >= D1) { yy -= D1; sy = c.box[1]; }
+      // kept x offsets form one interval [dxl, dxh]
+      int dxl = 1, dxh = 0;
       for (int dx = -R; dx <= R; ++dx) {
-        float gx = dx > 0 ? (float)(dx - 1) * w0 + hi0 : (dx < 0 ? (float)(-dx - 1) * w0 + lo0 : 0.0f);
-        gx = fmaxf(gx, 0.0f);
-        if (gx * gx + gyz > cull) continue;
-        int xx = cx + dx;
-        xx = xx < 0 ? xx + D0 : (xx >= D0 ? xx - D0 : xx);
-        uint32_t nc = (uint32_t)xx + c.dims[0] * ((uint32_t)yy + c.dims[1] * (uint32_t)zz);
-        uint32_t k0 = b.cell_start[nc], k1 = b.cell_start[nc + 1];
-        for (uint32_t k = k0; k < k1; ++k) {
-          if (k == t) continue;
-          float4 xj = b.pos[k];
-          // cheap reject, then the exact test of the reference
-          float ax = xi.x - xj.x, ay = xi.y - xj.y, az = xi.z - xj.z;
-          ax -= c.box[0] * rintf(ax * ib0);
-          ay -= c.box[1] * rintf(ay * ib1);
-          az -= c.box[2] * rintf(az * ib2);
-          if (ax * ax + ay * ay + az * az > lr_hi) continue;
-          if (dist_f(c, xi, xj) < lr) {
-            if (count < cap) {
-              b.vl[(size_t)count * n + t] = k;
-              count++;
-            } else {
-              overflow = 1;
-            }
+        float gx = fmaxf(dx > 0 ? (float)(dx - 1) * w0 + hi0 : (dx < 0 ? (float)(-dx - 1) * w0 + lo0 : 0.0f), 0.0f);
+        if (gx * gx + gyz <= cull) {
+          if (dxl > dxh) dxl = dx;
+          dxh = dx;
+        }
+      }
+      if (dxl > dxh) continue;
+      const uint32_t rowbase = c.dims[0] * ((uint32_t)yy + c.dims[1] * (uint32_t)zz);
+      // up to two slot ranges in canonical order, each with its own x image shift; dx = 0 is always
+      // kept (its gap is 0), so dxl <= 0 <= dxh and at most one end of the interval wraps
+      const int xa = cx + dxl, xb = cx + dxh;
+      int seg_lo[2], seg_hi[2];
+      float seg_sx[2];
+      int nseg;
+      if (xa < 0) {
+        seg_lo[0] = xa + D0; seg_hi[0] = D0 - 1; seg_sx[0] = -c.box[0];
+        seg_lo[1] = 0; seg_hi[1] = xb; seg_sx[1] = 0.0f;
+        nseg = 2;
+      } else if (xb >= D0) {
+        seg_lo[0] = xa; seg_hi[0] = D0 - 1; seg_sx[0] = 0.0f;
+        seg_lo[1] = 0; seg_hi[1] = xb - D0; seg_sx[1] = c.box[0];
+        nseg = 2;
+      } else {
+        seg_lo[0] = xa; seg_hi[0] = xb; seg_sx[0] = 0.0f;
+        nseg = 1;
+      }
+      for (int sg = 0; sg < nseg; ++sg) {
+        const uint32_t k0 = b.cell_start[rowbase + (uint32_t)seg_lo[sg]];
+        const uint32_t k1 = b.cell_start[rowbase + (uint32_t)seg_hi[sg] + 1u];
+        const float sx = seg_sx[sg];
+        for (uint32_t base = k0; base < k1; base += WAVE) {
+          const uint32_t k = base + lane;
+          bool hit = false;
+          if (k < k1 && k != t) {
+            float4 xj = b.pos[k];
+            // cheap reject through the known image shift, then the reference's exact test
+            float ax = (xj.x + sx) - xi.x, ay = (xj.y + sy) - xi.y, az = (xj.z + sz) - xi.z;
+            float a2 = ax * ax + ay * ay + az * az;
+            hit = a2 <= sure;
+            if (!hit && a2 <= cull) hit = dist_f(c, xi, xj) < lr;
           }
+          const uint64_t mask = __ballot(hit);
+          if (hit) {
+            uint32_t p = count + (uint32_t)__popcll(mask & below);
+            if (p < cap) b.vl[(size_t)p * n + t] = k;
+          }
+          count += (uint32_t)__popcll(mask);
         }
       }
     }
   }
-  b.vcount[t] = count;
-  if (overflow) b.flags[0] = 1u;
+  if (lane == 0) {
+    b.vcount[t] = count < cap ? count : cap;
+    if (count > cap) b.flags[0] = 1u;
+  }
 }
 
 // ================================================================================================
@@ -346,7 +385,7 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
   const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   if (h >= nh) return;
   const float4 xi = b.pos[b.head_slot[h]];
-  const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f);
+  const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f), lr_lo = (lr - 0.01f) * (lr - 0.01f);
   const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
   const uint32_t cap = c.es_cap;
   uint32_t count = 0;
@@ -359,12 +398,14 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
       ax -= c.box[0] * rintf(ax * ib0);
       ay -= c.box[1] * rintf(ay * ib1);
       az -= c.box[2] * rintf(az * ib2);
-      if (ax * ax + ay * ay + az * az <= lr_hi) hit = dist_f(c, xi, xj) < lr;
+      float a2 = ax * ax + ay * ay + az * az;
+      hit = a2 <= lr_lo;
+      if (!hit && a2 <= lr_hi) hit = dist_f(c, xi, xj) < lr;
     }
     uint64_t mask = __ballot(hit);
     if (hit) {
       uint32_t p = count + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (p < cap) b.es[(size_t)h * cap + p] = g;
+      if (p < cap) b.es[(size_t)h * cap + p] = b.head_slot[g];
     }
     count += (uint32_t)__popcll(mask);
   }
@@ -379,11 +420,10 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
 // sum of nonbondedSoup (pair.wgsl:168-204), then soup_es_force_far_main (electrostatics-long.wgsl
 // :63-82). Per-kind-pair constants are staged in LDS.
 // ================================================================================================
+// per kind pair: x = wca_cut(pairB)^2, y = pairB^2, z = class-table depth, w = 1 if both beads are
+// polar (acid-soap eligible); one 16-byte LDS read per pair
 struct PairTab {
-  float b2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];    // pairB^2
-  float cut2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];  // wca_cut(pairB)^2
-  float attr[SOUP_MAX_KINDS * SOUP_MAX_KINDS];  // class-table depth
-  float polar2[SOUP_MAX_KINDS * SOUP_MAX_KINDS];// 1 if both beads are polar (acid-soap eligible)
+  float4 e[SOUP_MAX_KINDS * SOUP_MAX_KINDS];
 };
 
 __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
@@ -391,10 +431,8 @@ __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
     uint32_t ki = k / SOUP_MAX_KINDS, kj = k % SOUP_MAX_KINDS;
     float bb = c.sigma * (c.radius[ki] + c.radius[kj]) * 0.5f;
     float cut = c.wca_factor * bb;
-    T.b2[k] = bb * bb;
-    T.cut2[k] = cut * cut;
-    T.attr[k] = c.attr[c.cls[ki]][c.cls[kj]];
-    T.polar2[k] = (c.cls[ki] == 1u && c.cls[kj] == 1u) ? 1.0f : 0.0f;
+    T.e[k] = make_float4(cut * cut, bb * bb, c.attr[c.cls[ki]][c.cls[kj]],
+                         (c.cls[ki] == 1u && c.cls[kj] == 1u) ? 1.0f : 0.0f);
   }
   __syncthreads();
 }
@@ -402,7 +440,7 @@ __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
 struct PairConst {
   float bx, by, bz, hx, hy, hz;
   float eps24, rc_attr, rc2_lo, rc2_hi, attr_k2, attr_a;
-  float acid, es_A, es_kappa, es_Frc, split2, es_rc;
+  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2;
 };
 
 __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
@@ -419,6 +457,12 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   p.es_A = c.es_A; p.es_kappa = c.es_kappa; p.es_Frc = c.es_Frc;
   p.split2 = c.es_split * c.es_split;
   p.es_rc = c.es_rc;
+  // no term acts at or beyond max(wca cuts, attraction shell, DH split): list entries there are skin
+  float wmax = 0.0f;
+  for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) wmax = fmaxf(wmax, c.radius[k]);
+  float wca_max = c.wca_factor * c.sigma * wmax;
+  float reach = fmaxf(fmaxf(wca_max, p.rc_attr + c.wc), c.es_split);
+  p.reach2 = reach * reach * 1.0001f;
   return p;
 }
 
@@ -436,46 +480,56 @@ struct Acc {
   double x, y, z;
 };
 
-// Accumulates the force on i from j (nonbondedSoup). The WCA core, where pair forces reach 10^3
-// while the net force is O(10), is evaluated in FP64; the smooth attraction and DH terms in f32;
-// the sum is FP64.
+// sin(x) for x in [0, pi]: reflected to [0, pi/2], odd Taylor polynomial through x^11
+// (truncation < 6e-8 at pi/2)
+__device__ __forceinline__ float sin_0_pi(float x) {
+#pragma clang fp contract(fast)
+  float y = x > 1.57079633f ? 3.14159265f - x : x;
+  float y2 = y * y;
+  float p = -2.50521084e-8f;
+  p = p * y2 + 2.75573192e-6f;
+  p = p * y2 - 1.98412698e-4f;
+  p = p * y2 + 8.33333333e-3f;
+  p = p * y2 - 1.66666667e-1f;
+  return y + y * y2 * p;
+}
+
+// Accumulates the force on i from j (nonbondedSoup). Per-pair terms in f32 with hardware reciprocal
+// and square root (1 ulp), branch-free WCA and attraction shell, the rare DH-near term behind a
+// branch, FP64 sum. The displacement is a - b; only a pair straddling the periodic boundary (|d| >
+// L/2 on some axis) takes the exact image of mi_acc, so the result stays within 1e-5 of the FP64
+// recomputation relative to max(|F_i|, median |F|).
 __device__ __forceinline__ void pair_force(const PairConst& P, const PairTab& T, float4 xi, float4 xj, float qi, float qj,
                                            Acc& f) {
 #pragma clang fp contract(fast)
-  float dx = mi_acc(xi.x, xj.x, P.bx, P.hx);
-  float dy = mi_acc(xi.y, xj.y, P.by, P.hy);
-  float dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
-  float r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 < 1e-12f) return;
-  uint32_t k = ((uint32_t)xi.w * SOUP_MAX_KINDS + (uint32_t)xj.w) & (SOUP_MAX_KINDS * SOUP_MAX_KINDS - 1u);
-  double fr = 0.0;  // (force along d) / r
-  if (r2 < T.cut2[k]) {
-    double d2 = (double)dx * dx + (double)dy * dy + (double)dz * dz;
-    double ir2 = 1.0 / d2;
-    double s2 = (double)T.b2[k] * ir2;
-    double s6 = s2 * s2 * s2;
-    fr = (double)P.eps24 * (2.0 * s6 * s6 - s6) * ir2;
+  float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+  if (fabsf(dx) > P.hx || fabsf(dy) > P.hy || fabsf(dz) > P.hz) {
+    dx = mi_acc(xi.x, xj.x, P.bx, P.hx);
+    dy = mi_acc(xi.y, xj.y, P.by, P.hy);
+    dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
   }
-  float as = T.attr[k];
-  if (P.acid > 0.0f && T.polar2[k] != 0.0f && ((qi != 0.0f) != (qj != 0.0f))) as += P.acid;
-  float qq = qi * qj;
+  float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
+  const float4 tb = T.e[((uint32_t)xi.w * SOUP_MAX_KINDS + (uint32_t)xj.w) & (SOUP_MAX_KINDS * SOUP_MAX_KINDS - 1u)];
+  float ir2 = __builtin_amdgcn_rcpf(r2);
+  float s2 = tb.y * ir2;
+  float s6 = s2 * s2 * s2;
+  float fr = (r2 < tb.x) ? P.eps24 * s6 * (2.0f * s6 - 1.0f) * ir2 : 0.0f;
+  float as = tb.z + ((P.acid > 0.0f && tb.w != 0.0f && ((qi != 0.0f) != (qj != 0.0f))) ? P.acid : 0.0f);
+  float r = __builtin_amdgcn_sqrtf(r2);
+  float ir = __builtin_amdgcn_rcpf(r);
+  float x = fminf(fmaxf(P.attr_k2 * (r - P.rc_attr), 0.0f), 3.14159265f);
   bool shell = as > 0.0f && r2 >= P.rc2_lo && r2 <= P.rc2_hi;
-  bool dh = qq != 0.0f && r2 < P.split2;
-  if (shell || dh) {
-    float r = sqrtf(r2), ir = 1.0f / r;
-    float g = 0.0f;
-    if (shell) g -= as * P.attr_a * sinf(P.attr_k2 * (r - P.rc_attr)) * ir;
-    if (dh && r < P.es_rc) {
-      float u = P.es_A * expf(-P.es_kappa * r) * ir;
-      g += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
-    }
-    fr += (double)g;
+  fr -= shell ? as * P.attr_a * sin_0_pi(x) * ir : 0.0f;
+  float qq = qi * qj;
+  if (qq != 0.0f && r2 < P.split2 && r < P.es_rc) {
+    float u = P.es_A * expf(-P.es_kappa * r) * ir;
+    fr += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
   }
-  if (fr != 0.0) {
-    f.x += fr * (double)dx;
-    f.y += fr * (double)dy;
-    f.z += fr * (double)dz;
-  }
+  if (r2 < 1e-12f) fr = 0.0f;  // r < 1e-6: no force (pair.wgsl:48)
+  double g = (double)fr;
+  f.x += g * (double)dx;
+  f.y += g * (double)dy;
+  f.z += g * (double)dz;
 }
 
 __device__ __forceinline__ void bond_term(const SoupConfig& c, const PairConst& P, float4 xi, float4 xp, bool fene, Acc& f) {
@@ -519,33 +573,52 @@ __device__ __forceinline__ void bonded_force(const SoupConfig& c, const PairCons
   if (owner != SOUP_NONE) bond_term(c, P, xi, b.pos[b.inv[owner]], true, f);
 }
 
-__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b) {
+__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int do_drift) {
   __shared__ PairTab T;
+  __shared__ uint32_t s_drift[256 / WAVE];
   load_pair_tab(c, T);
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= c.n) return;
-  const PairConst P = make_pair_const(c);
-  float4 xi = b.pos[t];
-  float qi = b.charge[t];
-  Acc f{0.0, 0.0, 0.0};
-  bonded_force(c, P, b, t, xi, f);
-  const uint32_t cnt = b.vcount[t], n = c.n;
-  const uint32_t* row = b.vl + t;
-  uint32_t s = 0;
-  for (; s + 4 <= cnt; s += 4) {
-    uint32_t u0 = row[(size_t)s * n], u1 = row[(size_t)(s + 1) * n], u2 = row[(size_t)(s + 2) * n], u3 = row[(size_t)(s + 3) * n];
-    float4 x0 = b.pos[u0], x1 = b.pos[u1], x2 = b.pos[u2], x3 = b.pos[u3];
-    float q0 = b.charge[u0], q1 = b.charge[u1], q2 = b.charge[u2], q3 = b.charge[u3];
-    pair_force(P, T, xi, x0, qi, q0, f);
-    pair_force(P, T, xi, x1, qi, q1, f);
-    pair_force(P, T, xi, x2, qi, q2, f);
-    pair_force(P, T, xi, x3, qi, q3, f);
+  const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
+  uint32_t drift_bits = 0;
+  if (t < c.n) {
+    const PairConst P = make_pair_const(c);
+    float4 xi = b.pos[t];
+    float qi = b.charge[t];
+    Acc f{0.0, 0.0, 0.0};
+    bonded_force(c, P, b, t, xi, f);
+    const uint32_t cnt = b.vcount[t], n = c.n;
+    const uint32_t* row = b.vl + t;
+    uint32_t s = 0;
+    for (; s + 4 <= cnt; s += 4) {
+      uint32_t u0 = row[(size_t)s * n], u1 = row[(size_t)(s + 1) * n], u2 = row[(size_t)(s + 2) * n], u3 = row[(size_t)(s + 3) * n];
+      float4 x0 = b.pos[u0], x1 = b.pos[u1], x2 = b.pos[u2], x3 = b.pos[u3];
+      float q0 = b.charge[u0], q1 = b.charge[u1], q2 = b.charge[u2], q3 = b.charge[u3];
+      pair_force(P, T, xi, x0, qi, q0, f);
+      pair_force(P, T, xi, x1, qi, q1, f);
+      pair_force(P, T, xi, x2, qi, q2, f);
+      pair_force(P, T, xi, x3, qi, q3, f);
+    }
+    for (; s < cnt; ++s) {
+      uint32_t u = row[(size_t)s * n];
+      pair_force(P, T, xi, b.pos[u], qi, b.charge[u], f);
+    }
+    b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
+    if (do_drift) {
+      // soup_max_drift_main (verlet.wgsl:115-123), fused: same positions, same arithmetic
+      float4 x0 = b.pos_at_rebuild[t];
+      float dx = mi_f(xi.x - x0.x, c.box[0]), dy = mi_f(xi.y - x0.y, c.box[1]), dz = mi_f(xi.z - x0.z, c.box[2]);
+      drift_bits = __float_as_uint(dx * dx + dy * dy + dz * dz);
+    }
   }
-  for (; s < cnt; ++s) {
-    uint32_t u = row[(size_t)s * n];
-    pair_force(P, T, xi, b.pos[u], qi, b.charge[u], f);
+  if (do_drift) {
+    drift_bits = wave_max_u(drift_bits);
+    if ((threadIdx.x & (WAVE - 1)) == 0) s_drift[threadIdx.x / WAVE] = drift_bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t m = 0;
+      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = max(m, s_drift[w]);
+      if (m != 0u) atomicMax(&b.flags[2], m);
+    }
   }
-  b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
 }
 
 // O(N^2) reference force: soup_force_brute_main (step.wgsl:277-292), test path only
@@ -586,8 +659,9 @@ __device__ __forceinline__ void far_pair(const PairConst& P, float4 xi, float4 x
 // far DH from the head list, one wavefront per head
 __global__ __launch_bounds__(256) void k_es_far(SoupConfig c, DevBuf b, uint32_t nh) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
-  if (h >= nh) return;
+  const uint32_t k = (xcd_block() * blockDim.x + threadIdx.x) / WAVE;
+  if (k >= nh) return;
+  const uint32_t h = b.head_rows[k];
   const uint32_t t = b.head_slot[h];
   const float qi = b.charge[t];
   if (qi == 0.0f) return;
@@ -597,7 +671,7 @@ __global__ __launch_bounds__(256) void k_es_far(SoupConfig c, DevBuf b, uint32_t
   const uint32_t* row = b.es + (size_t)h * c.es_cap;
   Acc f{0.0, 0.0, 0.0};
   for (uint32_t s = lane; s < cnt; s += WAVE) {
-    uint32_t u = b.head_slot[row[s]];
+    uint32_t u = row[s];
     far_pair(P, xi, b.pos[u], qi, b.charge[u], f);
   }
   f.x = wave_sum_d(f.x);
@@ -653,7 +727,7 @@ __device__ __forceinline__ uint32_t role_of(const SoupConfig& c, uint32_t r, flo
 }
 
 __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   const float4 xi = b.pos[t];
   const float ti = xi.w;
@@ -1076,7 +1150,7 @@ __device__ __forceinline__ double pcg_next_u(uint32_t& s) {
   return (double)v / 4294967296.0;
 }
 
-__global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, uint32_t nh, double2* tab) {
+__global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, uint32_t nh, double2* tab, uint32_t* tab_g) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   if (h >= nh) return;
@@ -1086,8 +1160,9 @@ __global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, ui
   const double cut2 = c.ph_cutoff * c.ph_cutoff;
   const double reach = c.as_rc + c.as_wc, reach2 = reach * reach;
   for (uint32_t s = lane; s < cnt; s += WAVE) {
-    uint32_t g = b.es[(size_t)h * c.es_cap + s];
-    float4 xj = b.pos[b.head_slot[g]];
+    uint32_t u = b.es[(size_t)h * c.es_cap + s];
+    tab_g[(size_t)h * c.es_cap + s] = b.head_of_slot[u];
+    float4 xj = b.pos[u];
     double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
     dx = dx - js_round(dx / bx) * bx;
     dy = dy - js_round(dy / by) * by;
@@ -1117,7 +1192,8 @@ __global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, ui
 #define CHAIN_THREADS 256
 // stats: [0] attempts, [1] accepted, [2] deprotonated, [3] dEsCount; dstats: [0] dEsSum
 __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, DevBuf b, uint32_t nh, const double2* tab,
-                                                                uint32_t* rng_io, uint32_t* stats, double* dstats) {
+                                                                const uint32_t* tab_g, uint32_t* rng_io, uint32_t* stats,
+                                                                double* dstats) {
   extern __shared__ float qv[];  // charge of every head
   __shared__ double s_part[2][2][CHAIN_THREADS / WAVE];
   const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE, nw = CHAIN_THREADS / WAVE;
@@ -1133,7 +1209,7 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
     double hf = floor(u * (double)nh);
     uint32_t h = hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
     const uint32_t cnt = b.es_count[h];
-    const uint32_t* row = b.es + (size_t)h * c.es_cap;
+    const uint32_t* row = tab_g + (size_t)h * c.es_cap;
     const double2* trow = tab + (size_t)h * c.es_cap;
     double se = 0.0, sa = 0.0;
     for (uint32_t s = tid; s < cnt; s += CHAIN_THREADS) {
@@ -1221,14 +1297,15 @@ __global__ void k_vl_to_index(const uint32_t* vl, const uint32_t* vcount, const 
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   uint32_t i = id[t], cnt = vcount[t];
-  for (uint32_t s = 0; s < cap; ++s) out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)s * n + t]] : SOUP_NONE;
+  for (uint32_t s = 0; s < cap; ++s)
+    out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)s * n + t]] : SOUP_NONE;
 }
-__global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* head_id, uint32_t* out,
+__global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* id, uint32_t* out,
                               uint32_t nh, uint32_t cap) {
   uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   uint32_t cnt = es_count[h];
-  for (uint32_t s = 0; s < cap; ++s) out[(size_t)h * cap + s] = s < cnt ? head_id[es[(size_t)h * cap + s]] : SOUP_NONE;
+  for (uint32_t s = 0; s < cap; ++s) out[(size_t)h * cap + s] = s < cnt ? id[es[(size_t)h * cap + s]] : SOUP_NONE;
 }
 __global__ void k_vcount_to_index(const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -40,6 +40,7 @@ struct SoupHandle {
   DevBuf b{};
   DevBuf alt{};
   double2* ptab = nullptr;
+  uint32_t* ptab_g = nullptr;
   uint32_t* pstats = nullptr;  // [0..3] chain stats, [4] rng
   double* pdstats = nullptr;
   unsigned long long* lsum = nullptr;
@@ -113,6 +114,11 @@ static std::string fmt(const char* f, ...) {
   } while (0)
 
 static inline uint32_t blocks(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+// grid of an XCD-aware launch: a multiple of the 8 XCDs (kernels map blockIdx with xcd_block())
+static inline uint32_t xcd_blocks(uint64_t n, uint32_t bs) {
+  uint32_t b = blocks(n, bs);
+  return ((b + SOUP_XCDS - 1) / SOUP_XCDS) * SOUP_XCDS;
+}
 
 template <typename T>
 static int dalloc(SoupHandle* h, T** p, size_t count) {
@@ -189,23 +195,28 @@ static void ph_rebuild(SoupHandle* h) {
   prof_end(h, SOUP_PROF_SORT, e0);
   if (es_on(h)) {
     hipEvent_t e1 = prof_begin(h);
+    fill(h, h->b.head_of_slot, SOUP_NONE, n);
+    fill(h, h->b.head_flag, 0u, n);
     hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+    scan(h, h->b.head_flag, n, h->b.head_pos);
+    hipLaunchKernelGGL(k_head_rows, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
     hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
     prof_end(h, SOUP_PROF_ES_BUILD, e1);
   }
   hipEvent_t e2 = prof_begin(h);
-  hipLaunchKernelGGL(k_verlet_build, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_verlet_build, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_VERLET_BUILD, e2);
   fill(h, h->b.flags + 2, 0u, 1);
 }
 
-static void ph_force(SoupHandle* h) {
+// list force + far DH; with_drift also takes the drift max of soup_max_drift_main in the same pass
+static void ph_force(SoupHandle* h, bool with_drift = false) {
   hipEvent_t e = prof_begin(h);
-  hipLaunchKernelGGL(k_force_list, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b, with_drift ? 1 : 0);
   prof_end(h, SOUP_PROF_FORCE, e);
   if (es_on(h)) {
     hipEvent_t e1 = prof_begin(h);
-    hipLaunchKernelGGL(k_es_far, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    hipLaunchKernelGGL(k_es_far, dim3(xcd_blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
     prof_end(h, SOUP_PROF_ES_FAR, e1);
   }
 }
@@ -221,7 +232,7 @@ static void ph_bonds(SoupHandle* h) {
   hipEvent_t e = prof_begin(h);
   fill(h, h->b.flag_cc, 0u, n);
   fill(h, h->b.ev_flag, 0u, n);
-  hipLaunchKernelGGL(k_bond_propose, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_bond_propose, dim3(xcd_blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   hipLaunchKernelGGL(k_bond_filter, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   scan(h, h->b.ev_flag, n, h->b.ev_pos);
   hipLaunchKernelGGL(k_bond_events, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -235,8 +246,7 @@ static void one_step(SoupHandle* h) {
   const bool do_rebuild = (h->global_step % h->c.rebuild_every) == 0;
   ph_kick_drift(h);
   if (do_rebuild) ph_rebuild(h);
-  ph_force(h);
-  ph_drift(h);
+  ph_force(h, true);  // force + drift max (soup-integrate.ts:95-98)
   if (do_bonds) ph_bonds(h);
   ph_kick_thermo(h);
   h->global_step++;
@@ -311,13 +321,13 @@ static int assert_lists(SoupHandle* h) {
 static int sweep(SoupHandle* h, SoupSweepResult* out) {
   ph_rebuild(h);  // DH list at the current positions
   hipLaunchKernelGGL(k_proton_pairs, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh,
-                     h->ptab);
+                     h->ptab, h->ptab_g);
   fill(h, h->pstats, 0u, 4);
   HC(h, hipMemcpyAsync(h->pstats + 4, &h->prng, 4, hipMemcpyHostToDevice, h->stream));
   size_t lds = (size_t)h->nh * sizeof(float);
   hipEvent_t e = prof_begin(h);
   hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, h->ptab,
-                     h->pstats + 4, h->pstats, h->pdstats);
+                     h->ptab_g, h->pstats + 4, h->pstats, h->pdstats);
   prof_end(h, SOUP_PROF_PROTON, e);
   uint32_t st[5];
   double ds;
@@ -461,9 +471,14 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         uint32_t nh = h->nh > 0 ? h->nh : 1;
         DA(h, b.head_id, nh);
         DA(h, b.head_slot, nh);
+        DA(h, b.head_of_slot, n);
+        DA(h, b.head_flag, n);
+        DA(h, b.head_pos, (size_t)n + 1);
+        DA(h, b.head_rows, nh);
         if (h->nh > 0) {
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
+          DA(h, h->ptab_g, (size_t)h->nh * cfg->es_cap);
         } else {
           DA(h, b.es, 1);
         }
@@ -480,6 +495,10 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.pos_at_rebuild = b.pos_at_rebuild;
         a.cell_of = b.cell_of;
         a.inv = b.inv;
+        a.head_of_slot = b.head_of_slot;
+        a.head_flag = b.head_flag;
+        a.head_pos = b.head_pos;
+        a.head_rows = b.head_rows;
         // initial state: slot = particle index
         std::vector<uint4> bs(n);
         for (uint32_t i = 0; i < n; ++i)
@@ -647,7 +666,7 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
       break;
     case SOUP_FIELD_ES_LIST:
       hipLaunchKernelGGL(k_es_to_index, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b.es, h->b.es_count,
-                         h->b.head_id, reinterpret_cast<uint32_t*>(h->stage), h->nh, h->c.es_cap);
+                         h->b.id, reinterpret_cast<uint32_t*>(h->stage), h->nh, h->c.es_cap);
       break;
     case SOUP_FIELD_ES_COUNT: src = h->b.es_count; break;
     case SOUP_FIELD_HEAD_IDX: src = h->b.head_id; break;
