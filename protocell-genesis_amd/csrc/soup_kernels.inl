@@ -27,6 +27,27 @@ __device__ __forceinline__ double wave_sum_d(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
   return v;
 }
+// FP64 sum over the 64 lanes of a wavefront (all lanes active): butterfly inside each row of 16
+// lanes through DPP (no LDS round trip), then the four row sums read from lanes 0, 16, 32, 48
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ double wave_sum_d_dpp(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  v += dpp_d<0x140>(v);  // row_mirror
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uni_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
 __device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, WAVE));
   return v;
@@ -109,28 +130,52 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint32_t* in, u
   if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = block_sums[nb];
 }
 
+// |minimum-image displacement since the last rebuild|^2 as bits, uncontracted (verlet.wgsl:115-123)
+__device__ __forceinline__ uint32_t drift_sq_bits(const SoupConfig& c, float4 xi, float4 x0) {
+  const float ex = mi_f(xi.x - x0.x, c.box[0]), ey = mi_f(xi.y - x0.y, c.box[1]), ez = mi_f(xi.z - x0.z, c.box[2]);
+  return __float_as_uint(ex * ex + ey * ey + ez * ez);
+}
+
 // ================================================================================================
 // Integrator: kick_drift_wrap_main (soup/wgsl/step.wgsl:96-108),
 //             kick_thermostat_main (step.wgsl:110-142)
 // ================================================================================================
+// Also soup_max_drift_main (verlet.wgsl:115-123) on the new positions, which are the ones the force
+// of this step sees: one atomicMax per workgroup. On a rebuild step the rebuild clears the maximum
+// afterwards, as the reference's rebuild does before its own drift pass.
 __global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= c.n) return;
-  if (b.frozen[t] != 0u) {
-    b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
+  __shared__ uint32_t s_max[256 / WAVE];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bits = 0;
+  if (t < c.n) {
+    float4 x = b.pos[t];
+    if (b.frozen[t] != 0u) {
+      b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      float4 v = b.vel[t], f = b.force[t];
+      float hdt = 0.5f * c.dt;
+      v.x = v.x + hdt * f.x;
+      v.y = v.y + hdt * f.y;
+      v.z = v.z + hdt * f.z;
+      b.vel[t] = make_float4(v.x, v.y, v.z, 0.f);
+      float px = x.x + c.dt * v.x, py = x.y + c.dt * v.y, pz = x.z + c.dt * v.z;
+      px = px - floorf(px / c.box[0]) * c.box[0];
+      py = py - floorf(py / c.box[1]) * c.box[1];
+      pz = pz - floorf(pz / c.box[2]) * c.box[2];
+      x = make_float4(px, py, pz, x.w);
+      b.pos[t] = x;
+    }
+    bits = drift_sq_bits(c, x, b.pos_at_rebuild[t]);
   }
-  float4 v = b.vel[t], f = b.force[t], x = b.pos[t];
-  float hdt = 0.5f * c.dt;
-  v.x = v.x + hdt * f.x;
-  v.y = v.y + hdt * f.y;
-  v.z = v.z + hdt * f.z;
-  b.vel[t] = make_float4(v.x, v.y, v.z, 0.f);
-  float px = x.x + c.dt * v.x, py = x.y + c.dt * v.y, pz = x.z + c.dt * v.z;
-  px = px - floorf(px / c.box[0]) * c.box[0];
-  py = py - floorf(py / c.box[1]) * c.box[1];
-  pz = pz - floorf(pz / c.box[2]) * c.box[2];
-  b.pos[t] = make_float4(px, py, pz, x.w);
+  bits = wave_max_u(bits);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_max[threadIdx.x / WAVE] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = max(m, s_max[w]);
+    // the maximum only grows between rebuilds: most workgroups see a larger value and skip the atomic
+    if (m > __hip_atomic_load(&b.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&b.flags[2], m);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
@@ -167,19 +212,12 @@ __global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
 
 // soup_max_drift_main (verlet.wgsl:115-123): wave max, one atomic per wave.
 __global__ __launch_bounds__(256) void k_max_drift(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t bits = 0;
-  if (t < c.n) {
-    float4 x = b.pos[t], x0 = b.pos_at_rebuild[t];
-    float dx = mi_f(x.x - x0.x, c.box[0]), dy = mi_f(x.y - x0.y, c.box[1]), dz = mi_f(x.z - x0.z, c.box[2]);
-    float d2 = dx * dx + dy * dy + dz * dz;
-    bits = __float_as_uint(d2);
-  }
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bits = t < c.n ? drift_sq_bits(c, b.pos[t], b.pos_at_rebuild[t]) : 0u;
   bits = wave_max_u(bits);
   if ((threadIdx.x & (WAVE - 1)) == 0 && bits != 0u) atomicMax(&b.flags[2], bits);
 }
 
-// soup_scan_nonfinite_main (soup/wgsl/health.wgsl:52-68)
 __global__ __launch_bounds__(256) void k_nonfinite(SoupConfig c, DevBuf b) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t bp = 0, bv = 0;
@@ -301,7 +339,8 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
   const float cull = (lr + 0.01f) * (lr + 0.01f);
   const float sure = (lr - 0.01f) * (lr - 0.01f);  // inside this the exact f32 test cannot fail
   const int R = (int)c.walk_radius;
-  const uint32_t cap = c.list_cap, n = c.n;
+  const uint32_t cap = c.list_cap;
+  uint32_t* row = b.vl + (size_t)t * cap;
   const uint64_t below = (1ull << lane) - 1ull;
   uint32_t count = 0;
   for (int dz = -R; dz <= R; ++dz) {
@@ -363,7 +402,7 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
           const uint64_t mask = __ballot(hit);
           if (hit) {
             uint32_t p = count + (uint32_t)__popcll(mask & below);
-            if (p < cap) b.vl[(size_t)p * n + t] = k;
+            if (p < cap) row[p] = k;
           }
           count += (uint32_t)__popcll(mask);
         }
@@ -440,7 +479,7 @@ __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
 struct PairConst {
   float bx, by, bz, hx, hy, hz;
   float eps24, rc_attr, rc2_lo, rc2_hi, attr_k2, attr_a;
-  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2;
+  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2, reach;
 };
 
 __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
@@ -463,6 +502,7 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   float wca_max = c.wca_factor * c.sigma * wmax;
   float reach = fmaxf(fmaxf(wca_max, p.rc_attr + c.wc), c.es_split);
   p.reach2 = reach * reach * 1.0001f;
+  p.reach = reach;
   return p;
 }
 
@@ -573,51 +613,172 @@ __device__ __forceinline__ void bonded_force(const SoupConfig& c, const PairCons
   if (owner != SOUP_NONE) bond_term(c, P, xi, b.pos[b.inv[owner]], true, f);
 }
 
-__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int do_drift) {
+// minimum image of mi_acc written as selects (no divergent branch inside a wavefront)
+__device__ __forceinline__ float mi_sel(float a, float b, float L, float hL) {
+  const float d = a - b, lo = (a - L) - b, hi = a - (b - L);
+  return d > hL ? lo : (d < -hL ? hi : d);
+}
+
+// nonbondedSoup (pair.wgsl:168-204) for one pair already known to lie inside the interaction reach;
+// tb is the pair-table entry of the two kinds. f32 terms from one hardware reciprocal square root
+// (1 ulp), branch-free WCA and attraction shell, the DH-near term behind a branch, FP64 sum.
+__device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float dx, float dy, float dz, float qi, float qj,
+                                          Acc& f) {
+#pragma clang fp contract(fast)
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
+  const float ir2 = ir * ir;
+  const float s2 = tb.y * ir2;
+  const float s6 = s2 * s2 * s2;
+  float fr = (r2 < tb.x) ? P.eps24 * s6 * (2.0f * s6 - 1.0f) * ir2 : 0.0f;
+  const float as = tb.z + ((P.acid > 0.0f && tb.w != 0.0f && ((qi != 0.0f) != (qj != 0.0f))) ? P.acid : 0.0f);
+  const float x = fminf(fmaxf(P.attr_k2 * (r - P.rc_attr), 0.0f), 3.14159265f);
+  const bool shell = as > 0.0f && r2 >= P.rc2_lo && r2 <= P.rc2_hi;
+  fr -= shell ? as * P.attr_a * sin_0_pi(x) * ir : 0.0f;
+  const float qq = qi * qj;
+  if (qq != 0.0f && r2 < P.split2 && r < P.es_rc) {
+    const float u = P.es_A * expf(-P.es_kappa * r) * ir;
+    fr += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
+  }
+  if (r2 < 1e-12f) fr = 0.0f;  // r < 1e-6: no force (pair.wgsl:48)
+  const double g = (double)fr;
+  f.x += g * (double)dx;
+  f.y += g * (double)dy;
+  f.z += g * (double)dz;
+}
+
+// bondedForce split over four lanes: lane k < 3 takes bond slot k (FENE + the bends through that
+// partner), lane 3 the centre link
+__device__ __forceinline__ void bonded_lane(const SoupConfig& c, const PairConst& P, const DevBuf& b, uint32_t t, float4 xi,
+                                            uint4 own, uint32_t owner, uint32_t k, Acc& f) {
+  if (k < 3u) {
+    const uint32_t p = k == 0u ? own.x : (k == 1u ? own.y : own.z);
+    if (p == SOUP_NONE) return;
+    const uint32_t me = b.id[t], u = b.inv[p];
+    bond_term(c, P, xi, b.pos[u], true, f);
+    const uint4 ms = b.bs[u];
+    const uint32_t ml[3] = {ms.x, ms.y, ms.z};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const uint32_t o = ml[q];
+      if (o == SOUP_NONE || o == me) continue;
+      bond_term(c, P, xi, b.pos[b.inv[o]], false, f);
+    }
+  } else if (k == 3u && owner != SOUP_NONE) {
+    bond_term(c, P, xi, b.pos[b.inv[owner]], true, f);
+  }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define FORCE_Q 128u    // per-wavefront ring of in-reach pairs (< 64 pending + one sweep of 64)
+#ifndef FORCE_BATCH
+#define FORCE_BATCH 4u  // list chunks of 64 whose loads are issued together
+#endif
+
+// List force: soup_force_list_main (verlet.wgsl:132-149) = bondedForce + the list sum of
+// nonbondedSoup, one wavefront per particle. Stage 1 sweeps the particle's list row 64 entries at a
+// time and keeps only pairs inside the interaction reach (about 30% of a list built with a 1.5 sigma
+// skin), compacting them with a ballot into an LDS ring; stage 2 runs the full pair terms on 64
+// queued pairs at a time, so every lane of the expensive part does useful work. Lane partial sums
+// are FP64 and reduced across the wavefront. A particle farther than reach + 0.5 sigma from every
+// face has no pair that wraps inside the reach, so its stage 1 takes the plain difference (equal to
+// the minimum image for every pair that can act); the others take the exact image of mi_acc.
+__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b) {
+#pragma clang fp contract(fast)
   __shared__ PairTab T;
-  __shared__ uint32_t s_drift[256 / WAVE];
+  __shared__ float4 s_q[256 / WAVE][FORCE_Q];
+  __shared__ float s_qc[256 / WAVE][FORCE_Q];
   load_pair_tab(c, T);
-  const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
-  uint32_t drift_bits = 0;
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const uint32_t t = uni(xcd_block() * (blockDim.x / WAVE) + w);
   if (t < c.n) {
     const PairConst P = make_pair_const(c);
-    float4 xi = b.pos[t];
-    float qi = b.charge[t];
+    const uint32_t cap = c.list_cap;
+    const uint32_t* row = b.vl + (size_t)t * cap;
+    // the first batch of list entries is requested with the particle's own data, before its length
+    // is known (entries past the length are in the allocation and are never used)
+    uint32_t u[FORCE_BATCH];
+#pragma unroll
+    for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
+      const uint32_t s = k * WAVE + lane;
+      u[k] = s < cap ? row[s] : t;
+    }
+    const float4 xv = b.pos[t];
+    const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
+    const float qi = uni_f(b.charge[t]);
+    const uint32_t cnt = uni(b.vcount[t]);
+    const uint4 ov = b.bs[t];
+    const uint4 own = make_uint4(uni(ov.x), uni(ov.y), uni(ov.z), 0u);
+    const uint32_t owner = uni(b.clink[t]);
+    const float4* trow = &T.e[((uint32_t)xi.w & (SOUP_MAX_KINDS - 1u)) * SOUP_MAX_KINDS];
     Acc f{0.0, 0.0, 0.0};
-    bonded_force(c, P, b, t, xi, f);
-    const uint32_t cnt = b.vcount[t], n = c.n;
-    const uint32_t* row = b.vl + t;
-    uint32_t s = 0;
-    for (; s + 4 <= cnt; s += 4) {
-      uint32_t u0 = row[(size_t)s * n], u1 = row[(size_t)(s + 1) * n], u2 = row[(size_t)(s + 2) * n], u3 = row[(size_t)(s + 3) * n];
-      float4 x0 = b.pos[u0], x1 = b.pos[u1], x2 = b.pos[u2], x3 = b.pos[u3];
-      float q0 = b.charge[u0], q1 = b.charge[u1], q2 = b.charge[u2], q3 = b.charge[u3];
-      pair_force(P, T, xi, x0, qi, q0, f);
-      pair_force(P, T, xi, x1, qi, q1, f);
-      pair_force(P, T, xi, x2, qi, q2, f);
-      pair_force(P, T, xi, x3, qi, q3, f);
+    if ((own.x & own.y & own.z & owner) != SOUP_NONE && lane < 4u) bonded_lane(c, P, b, t, xi, own, owner, lane, f);
+    float4* q = s_q[w];
+    float* qc = s_qc[w];
+    const float m = P.reach + 0.5f;
+    const bool interior = xi.x >= m && xi.x <= P.bx - m && xi.y >= m && xi.y <= P.by - m && xi.z >= m && xi.z <= P.bz - m;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t qn = 0, qh = 0;
+    for (uint32_t base = 0;;) {
+      float4 xj[FORCE_BATCH];
+      float qj[FORCE_BATCH];
+#pragma unroll
+      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
+        const uint32_t uk = base + k * WAVE + lane < cnt ? u[k] : t;
+        xj[k] = b.pos[uk];
+        qj[k] = b.charge[uk];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
+        float dx, dy, dz;
+        if (interior) {
+          dx = xi.x - xj[k].x; dy = xi.y - xj[k].y; dz = xi.z - xj[k].z;
+        } else {
+          dx = mi_sel(xi.x, xj[k].x, P.bx, P.hx); dy = mi_sel(xi.y, xj[k].y, P.by, P.hy); dz = mi_sel(xi.z, xj[k].z, P.bz, P.hz);
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool in = base + k * WAVE + lane < cnt && r2 >= 1e-12f && r2 < P.reach2;
+        const uint64_t mask = __ballot(in);
+        if (in) {
+          const uint32_t p = (qn + (uint32_t)__popcll(mask & below)) & (FORCE_Q - 1u);
+          q[p] = make_float4(dx, dy, dz, __uint_as_float((uint32_t)xj[k].w & (SOUP_MAX_KINDS - 1u)));
+          qc[p] = qj[k];
+        }
+        qn += (uint32_t)__popcll(mask);
+        if (qn - qh >= WAVE) {
+          wave_lds_sync();
+          const uint32_t p = (qh + lane) & (FORCE_Q - 1u);
+          const float4 e = q[p];
+          pair_body(P, trow[__float_as_uint(e.w)], e.x, e.y, e.z, qi, qc[p], f);
+          qh += WAVE;
+        }
+      }
+      base += FORCE_BATCH * WAVE;
+      if (base >= cnt) break;
+#pragma unroll
+      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
+        const uint32_t s = base + k * WAVE + lane;
+        u[k] = s < cnt ? row[s] : t;
+      }
     }
-    for (; s < cnt; ++s) {
-      uint32_t u = row[(size_t)s * n];
-      pair_force(P, T, xi, b.pos[u], qi, b.charge[u], f);
+    if (qn != qh) {
+      wave_lds_sync();
+      if (lane < qn - qh) {
+        const uint32_t p = (qh + lane) & (FORCE_Q - 1u);
+        const float4 e = q[p];
+        pair_body(P, trow[__float_as_uint(e.w)], e.x, e.y, e.z, qi, qc[p], f);
+      }
     }
-    b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
-    if (do_drift) {
-      // soup_max_drift_main (verlet.wgsl:115-123), fused: same positions, same arithmetic
-      float4 x0 = b.pos_at_rebuild[t];
-      float dx = mi_f(xi.x - x0.x, c.box[0]), dy = mi_f(xi.y - x0.y, c.box[1]), dz = mi_f(xi.z - x0.z, c.box[2]);
-      drift_bits = __float_as_uint(dx * dx + dy * dy + dz * dz);
-    }
-  }
-  if (do_drift) {
-    drift_bits = wave_max_u(drift_bits);
-    if ((threadIdx.x & (WAVE - 1)) == 0) s_drift[threadIdx.x / WAVE] = drift_bits;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t m = 0;
-      for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = max(m, s_drift[w]);
-      if (m != 0u) atomicMax(&b.flags[2], m);
-    }
+    f.x = wave_sum_d_dpp(f.x);
+    f.y = wave_sum_d_dpp(f.y);
+    f.z = wave_sum_d_dpp(f.z);
+    if (lane == 0) b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
   }
 }
 
@@ -726,8 +887,11 @@ __device__ __forceinline__ uint32_t role_of(const SoupConfig& c, uint32_t r, flo
   return kind == c.kindA[r] ? (uint32_t)c.roleA[r] : (uint32_t)c.roleB[r];
 }
 
+// one wavefront per particle: the lanes test 64 list entries at a time, and ballots keep the first
+// catalyst within reach and the first SOUP_CAND_CAP candidates in list order, as the serial walk does
 __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
-  uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const uint32_t t = xcd_block() * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   if (t >= c.n) return;
   const float4 xi = b.pos[t];
   const float ti = xi.w;
@@ -735,53 +899,65 @@ __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
   const uint4 own = b.bs[t];
   const float rcat = wca_cut(c, pair_b(c, ti, c.catalyst_kind));
   uint32_t catNear = 0, catId = SOUP_NONE, nCand = 0;
-  uint32_t candJ[SOUP_CAND_CAP], candR[SOUP_CAND_CAP];
-  const uint32_t cnt = b.vcount[t], n = c.n;
-  for (uint32_t s = 0; s < cnt; ++s) {
-    uint32_t u = b.vl[(size_t)s * n + t];
-    float4 xj = b.pos[u];
-    float tj = xj.w;
-    if (catNear == 0u && tj == c.catalyst_kind) {
-      if (dist_f(c, xi, xj) < rcat) {
-        catNear = 1u;
-        catId = b.id[u];
+  uint32_t myJ = SOUP_NONE, myR = 0;  // candidate k lives in lane k
+  const uint32_t cnt = b.vcount[t];
+  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  for (uint32_t base = 0; base < cnt && (catNear == 0u || nCand < SOUP_CAND_CAP); base += WAVE) {
+    const uint32_t s = base + lane;
+    bool isCat = false, isCand = false;
+    uint32_t j = 0;
+    int r = -1;
+    if (s < cnt) {
+      const uint32_t u = row[s];
+      const float4 xj = b.pos[u];
+      const float tj = xj.w;
+      j = b.id[u];
+      if (tj == c.catalyst_kind) isCat = dist_f(c, xi, xj) < rcat;
+      if (j > i) {
+        r = match_rule(c, ti, tj);
+        if (r >= 0 && own.x != j && own.y != j && own.z != j) isCand = dist_f(c, xi, xj) < wca_cut(c, pair_b(c, ti, tj));
       }
     }
-    if (nCand < SOUP_CAND_CAP) {
-      uint32_t j = b.id[u];
-      if (j > i) {
-        int r = match_rule(c, ti, tj);
-        if (r >= 0 && own.x != j && own.y != j && own.z != j) {
-          if (dist_f(c, xi, xj) < wca_cut(c, pair_b(c, ti, tj))) {
-            candJ[nCand] = j;
-            candR[nCand] = (uint32_t)r;
-            nCand++;
-          }
-        }
-      }
+    const uint64_t cm = __ballot(isCat);
+    if (catNear == 0u && cm != 0ull) {
+      catId = (uint32_t)__shfl((int)j, __ffsll((unsigned long long)cm) - 1, WAVE);
+      catNear = 1u;
+    }
+    uint64_t dm = __ballot(isCand);
+    while (dm != 0ull && nCand < SOUP_CAND_CAP) {
+      const int k = __ffsll((unsigned long long)dm) - 1;
+      const uint32_t cj = (uint32_t)__shfl((int)j, k, WAVE), cr = (uint32_t)__shfl(r, k, WAVE);
+      if (lane == nCand) { myJ = cj; myR = cr; }
+      ++nCand;
+      dm &= dm - 1ull;
     }
   }
+  // bondFormDecide draws (bond-dispatch.wgsl:104-112), serial in candidate order on every lane
   uint32_t rng = b.brng[t], nacc = 0;
   for (uint32_t k = 0; k < nCand; ++k) {
-    uint32_t r = candR[k];
+    const uint32_t cj = (uint32_t)__shfl((int)myJ, (int)k, WAVE), r = (uint32_t)__shfl((int)myR, (int)k, WAVE);
     if (c.req_cat[r] > 0.5f && catNear == 0u) continue;
     rng = pcg_soup(rng);
     if (uniform01(rng) >= c.att_form[r]) continue;
     rng = pcg_soup(rng);
     if (uniform01(rng) >= c.acc_form[r]) continue;
-    b.props[(size_t)t * SOUP_CAND_CAP + nacc] = make_uint2(candJ[k], r);
-    nacc++;
-    // a propagation proposal may give i, j and the catalyst a centre link: mark them
-    float tj = b.pos[b.inv[candJ[k]]].w;
-    if (role_of(c, r, ti) == 0u && role_of(c, r, tj) == 0u) {
-      b.flag_cc[i] = 1u;
-      b.flag_cc[candJ[k]] = 1u;
-      if (catId != SOUP_NONE) b.flag_cc[catId] = 1u;
+    if (lane == 0) {
+      b.props[(size_t)t * SOUP_CAND_CAP + nacc] = make_uint2(cj, r);
+      // a propagation proposal may give i, j and the catalyst a centre link: mark them
+      const float tj = b.pos[b.inv[cj]].w;
+      if (role_of(c, r, ti) == 0u && role_of(c, r, tj) == 0u) {
+        b.flag_cc[i] = 1u;
+        b.flag_cc[cj] = 1u;
+        if (catId != SOUP_NONE) b.flag_cc[catId] = 1u;
+      }
     }
+    nacc++;
   }
-  b.brng[t] = rng;
-  b.nacc[t] = nacc;
-  b.cat_id[t] = catId;
+  if (lane == 0) {
+    b.brng[t] = rng;
+    b.nacc[t] = nacc;
+    b.cat_id[t] = catId;
+  }
 }
 
 // drops termination proposals that cannot succeed in any order (the carbon has no centre and no
@@ -1298,7 +1474,7 @@ __global__ void k_vl_to_index(const uint32_t* vl, const uint32_t* vcount, const 
   if (t >= n) return;
   uint32_t i = id[t], cnt = vcount[t];
   for (uint32_t s = 0; s < cap; ++s)
-    out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)s * n + t]] : SOUP_NONE;
+    out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)t * cap + s]] : SOUP_NONE;
 }
 __global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* id, uint32_t* out,
                               uint32_t nh, uint32_t cap) {

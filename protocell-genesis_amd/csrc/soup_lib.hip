@@ -209,10 +209,10 @@ static void ph_rebuild(SoupHandle* h) {
   fill(h, h->b.flags + 2, 0u, 1);
 }
 
-// list force + far DH; with_drift also takes the drift max of soup_max_drift_main in the same pass
-static void ph_force(SoupHandle* h, bool with_drift = false) {
+// list force + far DH
+static void ph_force(SoupHandle* h) {
   hipEvent_t e = prof_begin(h);
-  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b, with_drift ? 1 : 0);
+  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_FORCE, e);
   if (es_on(h)) {
     hipEvent_t e1 = prof_begin(h);
@@ -232,7 +232,7 @@ static void ph_bonds(SoupHandle* h) {
   hipEvent_t e = prof_begin(h);
   fill(h, h->b.flag_cc, 0u, n);
   fill(h, h->b.ev_flag, 0u, n);
-  hipLaunchKernelGGL(k_bond_propose, dim3(xcd_blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipLaunchKernelGGL(k_bond_propose, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
   hipLaunchKernelGGL(k_bond_filter, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   scan(h, h->b.ev_flag, n, h->b.ev_pos);
   hipLaunchKernelGGL(k_bond_events, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -244,9 +244,9 @@ static void ph_bonds(SoupHandle* h) {
 static void one_step(SoupHandle* h) {
   const bool do_bonds = (h->global_step % h->c.bond_interval) == 0;
   const bool do_rebuild = (h->global_step % h->c.rebuild_every) == 0;
-  ph_kick_drift(h);
+  ph_kick_drift(h);  // also the drift max of this step's positions (soup-integrate.ts:96-98)
   if (do_rebuild) ph_rebuild(h);
-  ph_force(h, true);  // force + drift max (soup-integrate.ts:95-98)
+  ph_force(h);
   if (do_bonds) ph_bonds(h);
   ph_kick_thermo(h);
   h->global_step++;
@@ -377,6 +377,8 @@ static int validate(const SoupConfig* c) {
       return fail(nullptr, SOUP_E_INVALID,
                   fmt("neighbour grid: dims=[%u,%u,%u] needs dims>=%u (2*effectiveWalkRadius+1) on all three axes",
                       c->dims[0], c->dims[1], c->dims[2], 2 * c->walk_radius + 1));
+  if (c->n >= (1u << 29))
+    return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: N=%u exceeds the 2^29 - 1 particles the kernels address", c->n));
   if (c->list_cap == 0 || c->rebuild_every == 0 || c->bond_interval == 0)
     return fail(nullptr, SOUP_E_INVALID, "soup_create: list capacity, rebuildEvery and bondAttemptInterval must be > 0");
   if (c->es_enabled && c->n_heads > 0 && c->es_cap == 0)
