@@ -63,6 +63,7 @@ struct DevBuf {
   uint32_t* head_flag;    // by slot: 1 for a head (scan input)
   uint32_t* head_pos;     // exclusive scan of head_flag (n + 1)
   uint32_t* head_rows;    // heads in slot (spatial) order
+  float4* head_xyz;       // by head: position at the last rebuild (compact copy for the DH list build)
   uint32_t* es;
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,

@@ -45,6 +45,12 @@ __device__ __forceinline__ double wave_sum_d_dpp(double v) {
   v += dpp_d<0x140>(v);  // row_mirror
   return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
+// makes this wavefront's own LDS writes visible to its other lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ __forceinline__ float uni_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
@@ -306,6 +312,7 @@ __global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
   if (h >= nh) return;
   uint32_t u = b.inv[b.head_id[h]];
   b.head_slot[h] = u;
+  b.head_xyz[h] = b.pos[u];
   b.head_of_slot[u] = h;
   b.head_flag[u] = 1u;
 }
@@ -320,16 +327,23 @@ __global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
 // Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92), one wavefront per particle.
 // Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Slots
 // are cell-major with x fastest, so the kept cells of one (dz, dy) row are one contiguous slot range
-// (two when the row wraps in x), visited in canonical order; the 64 lanes sweep that range, and a
-// ballot compacts the hits in order. Rows and cells lying entirely beyond listRange + 0.01 sigma of
-// the particle are skipped: they hold no entry, so the list is unchanged.
+// (two when the row wraps in x), visited in canonical order. Lane r first sets up row r (cells lying
+// entirely beyond listRange + 0.01 sigma of the particle are dropped: they hold no entry, so the list
+// is unchanged) and a scan over the lanes lays the kept ranges end to end; the wavefront then sweeps
+// that one dense candidate range, VL_BATCH chunks of 64 per round of loads, and a ballot compacts
+// the hits in order. Needs (2R + 1)^2 <= 64 rows.
 // ================================================================================================
+#define VL_BATCH 4u
 __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
-  const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t t = xcd_block() * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+  __shared__ uint32_t s_end[256 / WAVE][WAVE];  // inclusive prefix of kept row lengths
+  __shared__ uint4 s_seg[256 / WAVE][WAVE];     // first slot of range a, its length, first slot of range b
+  __shared__ float4 s_sh[256 / WAVE][WAVE];     // image shifts: x of range a, x of range b, y, z
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const uint32_t t = uni(xcd_block() * (blockDim.x / WAVE) + w);
   if (t >= c.n) return;
-  const float4 xi = b.pos[t];
-  const uint32_t cell = b.cell_of[t];
+  const float4 xv = b.pos[t];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
+  const uint32_t cell = uni(b.cell_of[t]);
   const int D0 = (int)c.dims[0], D1 = (int)c.dims[1], D2 = (int)c.dims[2];
   const int cx = (int)(cell % c.dims[0]), cy = (int)((cell / c.dims[0]) % c.dims[1]), cz = (int)(cell / (c.dims[0] * c.dims[1]));
   const float w0 = c.box[0] / (float)D0, w1 = c.box[1] / (float)D1, w2 = c.box[2] / (float)D2;
@@ -338,75 +352,100 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
   const float lr = c.list_range;
   const float cull = (lr + 0.01f) * (lr + 0.01f);
   const float sure = (lr - 0.01f) * (lr - 0.01f);  // inside this the exact f32 test cannot fail
-  const int R = (int)c.walk_radius;
+  const int R = (int)c.walk_radius, side = 2 * R + 1;
+  const uint32_t nrows = (uint32_t)(side * side);
+  // ---- row set-up, lane r = row (dz, dy) in canonical order ----
+  uint32_t k0a = 0, lena = 0, k0b = 0, lenb = 0;
+  float sxa = 0.0f, sxb = 0.0f, sy = 0.0f, sz = 0.0f;
+  if (lane < nrows) {
+    const int dz = (int)lane / side - R, dy = (int)lane % side - R;
+    const float gz = fmaxf(dz > 0 ? (float)(dz - 1) * w2 + hi2 : (dz < 0 ? (float)(-dz - 1) * w2 + lo2 : 0.0f), 0.0f);
+    const float gy = fmaxf(dy > 0 ? (float)(dy - 1) * w1 + hi1 : (dy < 0 ? (float)(-dy - 1) * w1 + lo1 : 0.0f), 0.0f);
+    const float gyz = gy * gy + gz * gz;
+    if (gyz <= cull) {
+      int zz = cz + dz, yy = cy + dy;
+      if (zz < 0) { zz += D2; sz = -c.box[2]; } else if (zz >= D2) { zz -= D2; sz = c.box[2]; }
+      if (yy < 0) { yy += D1; sy = -c.box[1]; } else if (yy >= D1) { yy -= D1; sy = c.box[1]; }
+      // kept x offsets form one interval [dxl, dxh] around dx = 0 (whose gap is 0)
+      int dxl = 0, dxh = 0;
+      for (int dx = 1; dx <= R; ++dx) {
+        const float gh = (float)(dx - 1) * w0 + hi0, gl = (float)(dx - 1) * w0 + lo0;
+        if (fmaxf(gh, 0.0f) * fmaxf(gh, 0.0f) + gyz <= cull) dxh = dx;
+        if (fmaxf(gl, 0.0f) * fmaxf(gl, 0.0f) + gyz <= cull) dxl = -dx;
+      }
+      const uint32_t rowbase = c.dims[0] * ((uint32_t)yy + c.dims[1] * (uint32_t)zz);
+      const int xa = cx + dxl, xb = cx + dxh;
+      // at most one end of the interval wraps; the wrapped cells come first in canonical order
+      int la, ha, lb = 0, hb = -1;
+      if (xa < 0) {
+        la = xa + D0; ha = D0 - 1; sxa = -c.box[0]; lb = 0; hb = xb;
+      } else if (xb >= D0) {
+        la = xa; ha = D0 - 1; lb = 0; hb = xb - D0; sxb = c.box[0];
+      } else {
+        la = xa; ha = xb;
+      }
+      k0a = b.cell_start[rowbase + (uint32_t)la];
+      lena = b.cell_start[rowbase + (uint32_t)ha + 1u] - k0a;
+      if (hb >= lb) {
+        k0b = b.cell_start[rowbase + (uint32_t)lb];
+        lenb = b.cell_start[rowbase + (uint32_t)hb + 1u] - k0b;
+      }
+    }
+  }
+  uint32_t incl = lena + lenb;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, WAVE);
+    if (lane >= (uint32_t)o) incl += y;
+  }
+  s_end[w][lane] = incl;
+  s_seg[w][lane] = make_uint4(k0a, lena, k0b, 0u);
+  s_sh[w][lane] = make_float4(sxa, sxb, sy, sz);
+  const uint32_t total = uni((uint32_t)__shfl((int)incl, WAVE - 1, WAVE));
+  wave_lds_sync();
+  // ---- dense sweep ----
   const uint32_t cap = c.list_cap;
   uint32_t* row = b.vl + (size_t)t * cap;
   const uint64_t below = (1ull << lane) - 1ull;
   uint32_t count = 0;
-  for (int dz = -R; dz <= R; ++dz) {
-    float gz = fmaxf(dz > 0 ? (float)(dz - 1) * w2 + hi2 : (dz < 0 ? (float)(-dz - 1) * w2 + lo2 : 0.0f), 0.0f);
-    int zz = cz + dz;
-    float sz = 0.0f;
-    if (zz < 0) { zz += D2; sz = -c.box[2]; } else if (zz >= D2) { zz -= D2; sz = c.box[2]; }
-    for (int dy = -R; dy <= R; ++dy) {
-      float gy = fmaxf(dy > 0 ? (float)(dy - 1) * w1 + hi1 : (dy < 0 ? (float)(-dy - 1) * w1 + lo1 : 0.0f), 0.0f);
-      float gyz = gy * gy + gz * gz;
-      if (gyz > cull) continue;
-      int yy = cy + dy;
-      float sy = 0.0f;
-      if (yy < 0) { yy += D1; sy = -c.box[1]; } else if (yy >= D1) { yy -= D1; sy = c.box[1]; }
-      // kept x offsets form one interval [dxl, dxh]
-      int dxl = 1, dxh = 0;
-      for (int dx = -R; dx <= R; ++dx) {
-        float gx = fmaxf(dx > 0 ? (float)(dx - 1) * w0 + hi0 : (dx < 0 ? (float)(-dx - 1) * w0 + lo0 : 0.0f), 0.0f);
-        if (gx * gx + gyz <= cull) {
-          if (dxl > dxh) dxl = dx;
-          dxh = dx;
-        }
+  for (uint32_t base = 0; base < total; base += VL_BATCH * WAVE) {
+    uint32_t kk[VL_BATCH];
+    float3 sh[VL_BATCH];
+    float4 xj[VL_BATCH];
+#pragma unroll
+    for (uint32_t k = 0; k < VL_BATCH; ++k) {
+      const uint32_t f = base + k * WAVE + lane;
+      kk[k] = t;
+      sh[k] = make_float3(0.0f, 0.0f, 0.0f);
+      if (f < total) {
+        uint32_t r = 0;  // number of rows that end at or before f = the row holding f
+#pragma unroll
+        for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
+          if (r + step <= nrows && s_end[w][r + step - 1] <= f) r += step;
+        const uint32_t loc = f - (r > 0 ? s_end[w][r - 1] : 0u);
+        const uint4 sg = s_seg[w][r];
+        const float4 sv = s_sh[w][r];
+        const bool in_a = loc < sg.y;
+        kk[k] = in_a ? sg.x + loc : sg.z + (loc - sg.y);
+        sh[k] = make_float3(in_a ? sv.x : sv.y, sv.z, sv.w);
       }
-      if (dxl > dxh) continue;
-      const uint32_t rowbase = c.dims[0] * ((uint32_t)yy + c.dims[1] * (uint32_t)zz);
-      // up to two slot ranges in canonical order, each with its own x image shift; dx = 0 is always
-      // kept (its gap is 0), so dxl <= 0 <= dxh and at most one end of the interval wraps
-      const int xa = cx + dxl, xb = cx + dxh;
-      int seg_lo[2], seg_hi[2];
-      float seg_sx[2];
-      int nseg;
-      if (xa < 0) {
-        seg_lo[0] = xa + D0; seg_hi[0] = D0 - 1; seg_sx[0] = -c.box[0];
-        seg_lo[1] = 0; seg_hi[1] = xb; seg_sx[1] = 0.0f;
-        nseg = 2;
-      } else if (xb >= D0) {
-        seg_lo[0] = xa; seg_hi[0] = D0 - 1; seg_sx[0] = 0.0f;
-        seg_lo[1] = 0; seg_hi[1] = xb - D0; seg_sx[1] = c.box[0];
-        nseg = 2;
-      } else {
-        seg_lo[0] = xa; seg_hi[0] = xb; seg_sx[0] = 0.0f;
-        nseg = 1;
+      xj[k] = b.pos[kk[k]];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < VL_BATCH; ++k) {
+      bool hit = false;
+      if (base + k * WAVE + lane < total && kk[k] != t) {
+        // cheap reject through the known image shift, then the reference's exact test
+        const float ax = (xj[k].x + sh[k].x) - xi.x, ay = (xj[k].y + sh[k].y) - xi.y, az = (xj[k].z + sh[k].z) - xi.z;
+        const float a2 = ax * ax + ay * ay + az * az;
+        hit = a2 <= sure;
+        if (!hit && a2 <= cull) hit = dist_f(c, xi, xj[k]) < lr;
       }
-      for (int sg = 0; sg < nseg; ++sg) {
-        const uint32_t k0 = b.cell_start[rowbase + (uint32_t)seg_lo[sg]];
-        const uint32_t k1 = b.cell_start[rowbase + (uint32_t)seg_hi[sg] + 1u];
-        const float sx = seg_sx[sg];
-        for (uint32_t base = k0; base < k1; base += WAVE) {
-          const uint32_t k = base + lane;
-          bool hit = false;
-          if (k < k1 && k != t) {
-            float4 xj = b.pos[k];
-            // cheap reject through the known image shift, then the reference's exact test
-            float ax = (xj.x + sx) - xi.x, ay = (xj.y + sy) - xi.y, az = (xj.z + sz) - xi.z;
-            float a2 = ax * ax + ay * ay + az * az;
-            hit = a2 <= sure;
-            if (!hit && a2 <= cull) hit = dist_f(c, xi, xj) < lr;
-          }
-          const uint64_t mask = __ballot(hit);
-          if (hit) {
-            uint32_t p = count + (uint32_t)__popcll(mask & below);
-            if (p < cap) row[p] = k;
-          }
-          count += (uint32_t)__popcll(mask);
-        }
+      const uint64_t mask = __ballot(hit);
+      if (hit) {
+        const uint32_t p = count + (uint32_t)__popcll(mask & below);
+        if (p < cap) row[p] = kk[k];
       }
+      count += (uint32_t)__popcll(mask);
     }
   }
   if (lane == 0) {
@@ -418,35 +457,49 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
 // ================================================================================================
 // DH head list: soup_build_es_list_main (electrostatics-long.wgsl:31-56), one wavefront per head,
 // candidates in ascending head order compacted with a ballot so the row keeps the reference order.
+// Candidate positions come from the compact head copy (L2-resident), ES_BATCH chunks of 64 per
+// round of loads.
 // ================================================================================================
+#define ES_BATCH 8u
 __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
+  const uint32_t h = uni((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   if (h >= nh) return;
-  const float4 xi = b.pos[b.head_slot[h]];
+  const float4 xv = b.head_xyz[h];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
   const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f), lr_lo = (lr - 0.01f) * (lr - 0.01f);
   const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
   const uint32_t cap = c.es_cap;
+  uint32_t* row = b.es + (size_t)h * cap;
+  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t count = 0;
-  for (uint32_t base = 0; base < nh; base += WAVE) {
-    uint32_t g = base + lane;
-    bool hit = false;
-    if (g < nh && g != h) {
-      float4 xj = b.pos[b.head_slot[g]];
-      float ax = xi.x - xj.x, ay = xi.y - xj.y, az = xi.z - xj.z;
-      ax -= c.box[0] * rintf(ax * ib0);
-      ay -= c.box[1] * rintf(ay * ib1);
-      az -= c.box[2] * rintf(az * ib2);
-      float a2 = ax * ax + ay * ay + az * az;
-      hit = a2 <= lr_lo;
-      if (!hit && a2 <= lr_hi) hit = dist_f(c, xi, xj) < lr;
+  for (uint32_t base = 0; base < nh; base += ES_BATCH * WAVE) {
+    float4 xj[ES_BATCH];
+#pragma unroll
+    for (uint32_t k = 0; k < ES_BATCH; ++k) {
+      const uint32_t g = base + k * WAVE + lane;
+      xj[k] = b.head_xyz[g < nh ? g : h];
     }
-    uint64_t mask = __ballot(hit);
-    if (hit) {
-      uint32_t p = count + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (p < cap) b.es[(size_t)h * cap + p] = b.head_slot[g];
+#pragma unroll
+    for (uint32_t k = 0; k < ES_BATCH; ++k) {
+      const uint32_t g = base + k * WAVE + lane;
+      bool hit = false;
+      if (g < nh && g != h) {
+        float ax = xi.x - xj[k].x, ay = xi.y - xj[k].y, az = xi.z - xj[k].z;
+        ax -= c.box[0] * rintf(ax * ib0);
+        ay -= c.box[1] * rintf(ay * ib1);
+        az -= c.box[2] * rintf(az * ib2);
+        const float a2 = ax * ax + ay * ay + az * az;
+        hit = a2 <= lr_lo;
+        if (!hit && a2 <= lr_hi) hit = dist_f(c, xi, xj[k]) < lr;
+      }
+      const uint64_t mask = __ballot(hit);
+      if (hit) {
+        const uint32_t p = count + (uint32_t)__popcll(mask & below);
+        if (p < cap) row[p] = b.head_slot[g];
+      }
+      count += (uint32_t)__popcll(mask);
     }
-    count += (uint32_t)__popcll(mask);
   }
   if (lane == 0) {
     b.es_count[h] = count < cap ? count : cap;
@@ -670,11 +723,6 @@ __device__ __forceinline__ void bonded_lane(const SoupConfig& c, const PairConst
   }
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 #define FORCE_Q 128u    // per-wavefront ring of in-reach pairs (< 64 pending + one sweep of 64)
 #ifndef FORCE_BATCH

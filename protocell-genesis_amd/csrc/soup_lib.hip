@@ -377,6 +377,9 @@ static int validate(const SoupConfig* c) {
       return fail(nullptr, SOUP_E_INVALID,
                   fmt("neighbour grid: dims=[%u,%u,%u] needs dims>=%u (2*effectiveWalkRadius+1) on all three axes",
                       c->dims[0], c->dims[1], c->dims[2], 2 * c->walk_radius + 1));
+  if (c->walk_radius > 3)
+    return fail(nullptr, SOUP_E_INVALID,
+                fmt("soup_create: list walk radius %u exceeds 3 (the list build sets up one cell row per lane)", c->walk_radius));
   if (c->n >= (1u << 29))
     return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: N=%u exceeds the 2^29 - 1 particles the kernels address", c->n));
   if (c->list_cap == 0 || c->rebuild_every == 0 || c->bond_interval == 0)
@@ -477,6 +480,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.head_flag, n);
         DA(h, b.head_pos, (size_t)n + 1);
         DA(h, b.head_rows, nh);
+        DA(h, b.head_xyz, nh);
         if (h->nh > 0) {
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
@@ -501,6 +505,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_flag = b.head_flag;
         a.head_pos = b.head_pos;
         a.head_rows = b.head_rows;
+        a.head_xyz = b.head_xyz;
         // initial state: slot = particle index
         std::vector<uint4> bs(n);
         for (uint32_t i = 0; i < n; ++i)
