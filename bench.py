@@ -8,10 +8,12 @@ the cold-start minimiser. Inputs are resident in HBM before the timed region; th
 exactly --steps integration steps of SoupSystem.step (with the per-1000-step guards and the sweeps
 that fall inside it).
 
-Prints ONE JSON line: steps/s, the roofline of the dominant kernel (the list force kernel, HBM
-bound; algorithmic bytes per particle = 56 + 4 * mean Verlet length) measured with HIP events on the
-library's stream, and the CPU baseline: the oracle's OpenMP step on the host cores, timed on a
-bounded sample of the same state.
+Prints ONE JSON line: steps/s, the roofline of the dominant kernel (the list force kernel, which
+also sums the far DH rows of charged heads; algorithmic bytes per launch = N * (56 + 4 * mean Verlet
+length) + heads * (4 + 4 * mean DH length)) with its average duration measured by HIP events on the
+library's stream over the timed region, and the CPU baseline: the oracle's OpenMP step on the host
+cores, timed on a bounded sample of the same state. A separate fully bracketed run of
+--phase-steps steps gives the per-phase breakdown.
 """
 import argparse
 import json
@@ -75,6 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-sample", type=int, default=20)
+    ap.add_argument("--phase-steps", type=int, default=1000, help="steps of the separate per-phase profile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale reference) and exit")
@@ -116,7 +119,9 @@ def main():
         done += k
     log(f"warmup {args.warmup} steps done ({time.perf_counter() - t0:.1f} s)")
 
-    s.profile(True)
+    # timed region: HIP events bracket only the force kernel (the roofline's dominant kernel), on the
+    # library's own stream
+    s.profile(True, phases=["force"])
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     done = 0
@@ -128,19 +133,32 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     prof = s.profileRead()
+    steps_per_s = args.steps / elapsed
+
+    # phase breakdown from a separate, fully bracketed run (its events cost a little time per step)
+    s.profile(True)
+    s.step(args.phase_steps)
+    phases = s.profileRead()
     s.profile(False)
     occ = s.listOccupancyDEBUG()
-    steps_per_s = args.steps / elapsed
 
     f = prof["force"]
     force_ms = f["ms"] / max(1, f["count"])
     lbar = occ["verletMean"]
-    bytes_per_launch = R.N * (56.0 + 4.0 * lbar)
-    achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
     n_heads = R.es_heads
+    # algorithmic bytes of one force launch: per particle its own position, charge, count, bond slots,
+    # centre link and force (56 B) and 4 B per Verlet entry; per charged head its DH count and 4 B per
+    # DH entry; every neighbour gather is counted once per particle (the table is read, not re-read)
+    bytes_per_launch = R.N * (56.0 + 4.0 * lbar) + n_heads * (4.0 + 4.0 * occ["esMean"])
+    achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
     bytes_step = R.N * (245.4 + 4.6 * lbar) + n_heads * (40.0 + 4.4 * occ["esMean"])
     pair_per_step = occ["verletPairs"] + occ["esPairs"]
-    log("phase ms/launch: " + ", ".join(f"{k} {v['ms'] / max(1, v['count']):.3f}x{v['count']}" for k, v in prof.items()))
+    log("phase ms/launch: " + ", ".join(f"{k} {v['ms'] / max(1, v['count']):.3f}x{v['count']}" for k, v in phases.items() if v["count"]))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_force_list_campaign483k_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -167,7 +185,7 @@ def main():
             "pair_interactions_per_s": round(pair_per_step * steps_per_s, 1),
             "algorithmic_bytes_per_step": round(bytes_step),
             "step_roofline_frac": round(bytes_step * steps_per_s / (HBM_PEAK_GBS * 1e9), 4),
-            "phase_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in prof.items()},
+            "phase_ms_per_step": {k: round(v["ms"] / args.phase_steps, 4) for k, v in phases.items() if v["count"]},
         },
         "roofline": {
             "kernel": "k_force_list",
@@ -176,7 +194,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": round(force_ms, 4),
             "bytes_per_launch": round(bytes_per_launch),
         },

@@ -218,22 +218,23 @@ int soup_list_stats(SoupHandle* h, SoupListStats* out);
 int soup_relax(SoupHandle* h, uint32_t iterations, float d0);
 
 /* Device-time profile per phase, measured with HIP events on the handle's own stream around the
- * launches of each phase. soup_profile(h, 1) resets and starts, (h, 0) stops; soup_profile_read
- * fills SOUP_PROF_PHASES accumulated milliseconds and launch counts. */
+ * launches of each phase. soup_profile(h, mask) resets the totals and brackets the phases whose bit
+ * (1 << SOUP_PROF_x) is set; mask -1 brackets every phase, 0 stops. soup_profile_read fills
+ * SOUP_PROF_PHASES accumulated milliseconds and launch counts. */
 enum {
   SOUP_PROF_KICK_DRIFT = 0,
   SOUP_PROF_SORT = 1,          /* cell sort + state permutation of a rebuild */
   SOUP_PROF_ES_BUILD = 2,      /* DH head list build */
   SOUP_PROF_VERLET_BUILD = 3,  /* Verlet list build */
-  SOUP_PROF_FORCE = 4,         /* list force kernel */
-  SOUP_PROF_ES_FAR = 5,        /* far DH kernel */
+  SOUP_PROF_FORCE = 4,         /* list force kernel (near list + far DH rows of charged heads) */
+  SOUP_PROF_ES_FAR = 5,        /* far DH: fused into the force kernel, stays 0 */
   SOUP_PROF_DRIFT = 6,
   SOUP_PROF_BONDS = 7,         /* bond MC, all kernels */
   SOUP_PROF_KICK_THERMO = 8,
   SOUP_PROF_PROTON = 9,        /* constant-pH chain kernel */
   SOUP_PROF_PHASES = 16
 };
-int soup_profile(SoupHandle* h, int enable);
+int soup_profile(SoupHandle* h, int mask);
 int soup_profile_read(SoupHandle* h, double* ms, uint64_t* count);
 
 /* Library build/version string. */

@@ -14,8 +14,9 @@
 // ascending particle index inside each cell).
 //
 // The DH list is row-major, es[h * es_cap + s]: row h is the h-th charged-kind particle in ascending
-// particle-index order (electrostatics-long.wgsl:15-29); its entries are the SLOTS of the neighbour
-// heads, kept in ascending head order (the reference's order).
+// particle-index order (electrostatics-long.wgsl:15-29); its entries are the neighbour HEAD indices,
+// ascending (the reference's order). DH partners are always heads, so the force gathers them from the
+// compact head_q array (about 300 KB at campaign size, resident in L2) instead of the particle arrays.
 //
 // Launch geometry: workgroups are dispatched round-robin over the 8 XCDs, so the gather-heavy kernels
 // remap blockIdx so that each XCD owns one contiguous run of slots (a spatial slab) and its 4 MB L2
@@ -63,7 +64,7 @@ struct DevBuf {
   uint32_t* head_flag;    // by slot: 1 for a head (scan input)
   uint32_t* head_pos;     // exclusive scan of head_flag (n + 1)
   uint32_t* head_rows;    // heads in slot (spatial) order
-  float4* head_xyz;       // by head: position at the last rebuild (compact copy for the DH list build)
+  float4* head_q;         // by head: (x, y, z, charge), refreshed before every force evaluation
   uint32_t* es;
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,

@@ -312,12 +312,21 @@ __global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
   if (h >= nh) return;
   uint32_t u = b.inv[b.head_id[h]];
   b.head_slot[h] = u;
-  b.head_xyz[h] = b.pos[u];
+  const float4 x = b.pos[u];
+  b.head_q[h] = make_float4(x.x, x.y, x.z, b.charge[u]);
   b.head_of_slot[u] = h;
   b.head_flag[u] = 1u;
 }
 
 // heads in slot order: the far-force and pair-table kernels walk rows in this spatial order
+// head_q from the current positions and charges (every force evaluation)
+__global__ __launch_bounds__(256) void k_head_refresh(DevBuf b, uint32_t nh) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const uint32_t u = b.head_slot[h];
+  const float4 x = b.pos[u];
+  b.head_q[h] = make_float4(x.x, x.y, x.z, b.charge[u]);
+}
 __global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < c.n && b.head_flag[t]) b.head_rows[b.head_pos[t]] = b.head_of_slot[t];
@@ -457,15 +466,15 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
 // ================================================================================================
 // DH head list: soup_build_es_list_main (electrostatics-long.wgsl:31-56), one wavefront per head,
 // candidates in ascending head order compacted with a ballot so the row keeps the reference order.
-// Candidate positions come from the compact head copy (L2-resident), ES_BATCH chunks of 64 per
-// round of loads.
+// Candidate positions come from the compact head array (L2-resident), ES_BATCH chunks of 64 per
+// round of loads; entries are head indices.
 // ================================================================================================
 #define ES_BATCH 8u
 __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t h = uni((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
   if (h >= nh) return;
-  const float4 xv = b.head_xyz[h];
+  const float4 xv = b.head_q[h];
   const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
   const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f), lr_lo = (lr - 0.01f) * (lr - 0.01f);
   const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
 #pragma unroll
     for (uint32_t k = 0; k < ES_BATCH; ++k) {
       const uint32_t g = base + k * WAVE + lane;
-      xj[k] = b.head_xyz[g < nh ? g : h];
+      xj[k] = b.head_q[g < nh ? g : h];
     }
 #pragma unroll
     for (uint32_t k = 0; k < ES_BATCH; ++k) {
@@ -496,7 +505,7 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
       const uint64_t mask = __ballot(hit);
       if (hit) {
         const uint32_t p = count + (uint32_t)__popcll(mask & below);
-        if (p < cap) row[p] = b.head_slot[g];
+        if (p < cap) row[p] = g;
       }
       count += (uint32_t)__popcll(mask);
     }
@@ -532,7 +541,7 @@ __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
 struct PairConst {
   float bx, by, bz, hx, hy, hz;
   float eps24, rc_attr, rc2_lo, rc2_hi, attr_k2, attr_a;
-  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2, reach;
+  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2, reach, es_kappa_l2;
 };
 
 __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
@@ -547,6 +556,7 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   p.attr_a = c.epsilon * SOUP_PI_F / (2.0f * c.wc);
   p.acid = c.acid_soap;
   p.es_A = c.es_A; p.es_kappa = c.es_kappa; p.es_Frc = c.es_Frc;
+  p.es_kappa_l2 = -c.es_kappa * 1.44269504f;  // exp(-kappa r) = exp2(es_kappa_l2 * r)
   p.split2 = c.es_split * c.es_split;
   p.es_rc = c.es_rc;
   // no term acts at or beyond max(wca cuts, attraction shell, DH split): list entries there are skin
@@ -691,10 +701,28 @@ __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float d
   fr -= shell ? as * P.attr_a * sin_0_pi(x) * ir : 0.0f;
   const float qq = qi * qj;
   if (qq != 0.0f && r2 < P.split2 && r < P.es_rc) {
-    const float u = P.es_A * expf(-P.es_kappa * r) * ir;
+    const float u = P.es_A * __builtin_amdgcn_exp2f(P.es_kappa_l2 * r) * ir;
     fr += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
   }
   if (r2 < 1e-12f) fr = 0.0f;  // r < 1e-6: no force (pair.wgsl:48)
+  const double g = (double)fr;
+  f.x += g * (double)dx;
+  f.y += g * (double)dy;
+  f.z += g * (double)dz;
+}
+
+// one far DH pair of soup_es_force_far_main (electrostatics-long.wgsl:63-82): only pairs at or beyond
+// the near/far split and inside the DH cutoff act; f32 terms from one reciprocal square root and one
+// hardware exp2 (relative error ~1e-7 over the cutoff range), FP64 sum
+__device__ __forceinline__ void far_pair_fast(const PairConst& P, float4 xi, float4 xj, float qi, float qj, Acc& f) {
+#pragma clang fp contract(fast)
+  const float qq = qi * qj;
+  const float dx = mi_sel(xi.x, xj.x, P.bx, P.hx), dy = mi_sel(xi.y, xj.y, P.by, P.hy), dz = mi_sel(xi.z, xj.z, P.bz, P.hz);
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
+  const float u = P.es_A * __builtin_amdgcn_exp2f(P.es_kappa_l2 * r) * ir;
+  const float fr = (qq != 0.0f && r2 >= P.split2 && r < P.es_rc) ? qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir : 0.0f;
   const double g = (double)fr;
   f.x += g * (double)dx;
   f.y += g * (double)dy;
@@ -730,14 +758,15 @@ __device__ __forceinline__ void bonded_lane(const SoupConfig& c, const PairConst
 #endif
 
 // List force: soup_force_list_main (verlet.wgsl:132-149) = bondedForce + the list sum of
-// nonbondedSoup, one wavefront per particle. Stage 1 sweeps the particle's list row 64 entries at a
+// nonbondedSoup, plus soup_es_force_far_main for charged heads, one wavefront per particle. Stage 1 sweeps the particle's list row 64 entries at a
 // time and keeps only pairs inside the interaction reach (about 30% of a list built with a 1.5 sigma
 // skin), compacting them with a ballot into an LDS ring; stage 2 runs the full pair terms on 64
 // queued pairs at a time, so every lane of the expensive part does useful work. Lane partial sums
 // are FP64 and reduced across the wavefront. A particle farther than reach + 0.5 sigma from every
 // face has no pair that wraps inside the reach, so its stage 1 takes the plain difference (equal to
 // the minimum image for every pair that can act); the others take the exact image of mi_acc.
-__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b) {
+#define FAR_BATCH 4u
+__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int es) {
 #pragma clang fp contract(fast)
   __shared__ PairTab T;
   __shared__ float4 s_q[256 / WAVE][FORCE_Q];
@@ -823,6 +852,27 @@ __global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b) {
         pair_body(P, trow[__float_as_uint(e.w)], e.x, e.y, e.z, qi, qc[p], f);
       }
     }
+    // a charged head also sums its far DH row here, in the same FP64 accumulator, so its force is
+    // rounded to f32 once
+    const uint32_t hh = es ? uni(b.head_of_slot[t]) : SOUP_NONE;
+    if (hh != SOUP_NONE && qi != 0.0f) {
+      const uint32_t ecnt = uni(b.es_count[hh]);
+      const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+      for (uint32_t base = 0; base < ecnt; base += FAR_BATCH * WAVE) {
+        uint32_t v[FAR_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < FAR_BATCH; ++k) {
+          const uint32_t s = base + k * WAVE + lane;
+          v[k] = s < ecnt ? erow[s] : hh;
+        }
+        float4 xj[FAR_BATCH];
+#pragma unroll
+        for (uint32_t k = 0; k < FAR_BATCH; ++k) xj[k] = b.head_q[v[k]];
+#pragma unroll
+        for (uint32_t k = 0; k < FAR_BATCH; ++k)
+          far_pair_fast(P, xi, xj[k], qi, base + k * WAVE + lane < ecnt ? xj[k].w : 0.0f, f);
+      }
+    }
     f.x = wave_sum_d_dpp(f.x);
     f.y = wave_sum_d_dpp(f.y);
     f.z = wave_sum_d_dpp(f.z);
@@ -863,34 +913,6 @@ __device__ __forceinline__ void far_pair(const PairConst& P, float4 xi, float4 x
   f.x += fr * (double)dx;
   f.y += fr * (double)dy;
   f.z += fr * (double)dz;
-}
-
-// far DH from the head list, one wavefront per head
-__global__ __launch_bounds__(256) void k_es_far(SoupConfig c, DevBuf b, uint32_t nh) {
-  const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t k = (xcd_block() * blockDim.x + threadIdx.x) / WAVE;
-  if (k >= nh) return;
-  const uint32_t h = b.head_rows[k];
-  const uint32_t t = b.head_slot[h];
-  const float qi = b.charge[t];
-  if (qi == 0.0f) return;
-  const PairConst P = make_pair_const(c);
-  const float4 xi = b.pos[t];
-  const uint32_t cnt = b.es_count[h];
-  const uint32_t* row = b.es + (size_t)h * c.es_cap;
-  Acc f{0.0, 0.0, 0.0};
-  for (uint32_t s = lane; s < cnt; s += WAVE) {
-    uint32_t u = row[s];
-    far_pair(P, xi, b.pos[u], qi, b.charge[u], f);
-  }
-  f.x = wave_sum_d(f.x);
-  f.y = wave_sum_d(f.y);
-  f.z = wave_sum_d(f.z);
-  if (lane == 0) {
-    // the near part was rounded to f32 by the list force; add the far part in FP64 and round once
-    float4 g = b.force[t];
-    b.force[t] = make_float4((float)((double)g.x + f.x), (float)((double)g.y + f.y), (float)((double)g.z + f.z), 0.f);
-  }
 }
 
 // far DH over all pairs: soup_es_force_far_brute_main (electrostatics-long.wgsl:87-103), test only
@@ -1384,9 +1406,9 @@ __global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, ui
   const double cut2 = c.ph_cutoff * c.ph_cutoff;
   const double reach = c.as_rc + c.as_wc, reach2 = reach * reach;
   for (uint32_t s = lane; s < cnt; s += WAVE) {
-    uint32_t u = b.es[(size_t)h * c.es_cap + s];
-    tab_g[(size_t)h * c.es_cap + s] = b.head_of_slot[u];
-    float4 xj = b.pos[u];
+    const uint32_t g = b.es[(size_t)h * c.es_cap + s];
+    tab_g[(size_t)h * c.es_cap + s] = g;
+    const float4 xj = b.pos[b.head_slot[g]];
     double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
     dx = dx - js_round(dx / bx) * bx;
     dy = dy - js_round(dy / by) * by;
@@ -1524,12 +1546,13 @@ __global__ void k_vl_to_index(const uint32_t* vl, const uint32_t* vcount, const 
   for (uint32_t s = 0; s < cap; ++s)
     out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)t * cap + s]] : SOUP_NONE;
 }
-__global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* id, uint32_t* out,
+// DH list in the reference layout: entries are head indices, exported as particle indices
+__global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* head_id, uint32_t* out,
                               uint32_t nh, uint32_t cap) {
   uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   uint32_t cnt = es_count[h];
-  for (uint32_t s = 0; s < cap; ++s) out[(size_t)h * cap + s] = s < cnt ? id[es[(size_t)h * cap + s]] : SOUP_NONE;
+  for (uint32_t s = 0; s < cap; ++s) out[(size_t)h * cap + s] = s < cnt ? head_id[es[(size_t)h * cap + s]] : SOUP_NONE;
 }
 __global__ void k_vcount_to_index(const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

@@ -22,7 +22,7 @@ static thread_local std::string g_create_error;
 // Per-phase device timing with HIP events on the handle's stream (soup_profile): every bracketed
 // launch gets an event pair from a pool; pairs are resolved at the next synchronisation.
 struct Prof {
-  bool on = false;
+  uint32_t mask = 0;  // bit k: phase k is bracketed
   double ms[SOUP_PROF_PHASES] = {0};
   uint64_t count[SOUP_PROF_PHASES] = {0};
   std::vector<hipEvent_t> pool;
@@ -67,14 +67,14 @@ static hipEvent_t prof_event(SoupHandle* h) {
   return e;
 }
 // brackets the launches issued between prof_begin and prof_end
-static hipEvent_t prof_begin(SoupHandle* h) {
-  if (!h->prof.on) return nullptr;
+static hipEvent_t prof_begin(SoupHandle* h, int phase) {
+  if (!(h->prof.mask >> phase & 1u)) return nullptr;
   hipEvent_t a = prof_event(h);
   (void)hipEventRecord(a, h->stream);
   return a;
 }
 static void prof_end(SoupHandle* h, int phase, hipEvent_t a) {
-  if (!h->prof.on || !a) return;
+  if (!a) return;
   hipEvent_t b = prof_event(h);
   (void)hipEventRecord(b, h->stream);
   h->prof.pending.push_back({phase, a, b});
@@ -156,17 +156,17 @@ static bool es_on(const SoupHandle* h) { return h->c.es_enabled && h->nh > 0; }
 
 // ---- phases ------------------------------------------------------------------------------------
 static void ph_kick_drift(SoupHandle* h) {
-  hipEvent_t e = prof_begin(h);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_KICK_DRIFT);
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_KICK_DRIFT, e);
 }
 static void ph_kick_thermo(SoupHandle* h) {
-  hipEvent_t e = prof_begin(h);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_KICK_THERMO);
   hipLaunchKernelGGL(k_kick_thermo, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_KICK_THERMO, e);
 }
 static void ph_drift(SoupHandle* h) {
-  hipEvent_t e = prof_begin(h);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_DRIFT);
   hipLaunchKernelGGL(k_max_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_DRIFT, e);
 }
@@ -175,7 +175,7 @@ static void ph_drift(SoupHandle* h) {
 // drift reset -- here with the permutation of the state into the new cell order.
 static void ph_rebuild(SoupHandle* h) {
   const uint32_t n = h->n;
-  hipEvent_t e0 = prof_begin(h);
+  hipEvent_t e0 = prof_begin(h, SOUP_PROF_SORT);
   fill(h, h->b.cell_count, 0u, h->ncells);
   hipLaunchKernelGGL(k_cell_count, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   scan(h, h->b.cell_count, h->ncells, h->b.cell_start);
@@ -194,7 +194,7 @@ static void ph_rebuild(SoupHandle* h) {
   std::swap(h->b.id, h->alt.id);
   prof_end(h, SOUP_PROF_SORT, e0);
   if (es_on(h)) {
-    hipEvent_t e1 = prof_begin(h);
+    hipEvent_t e1 = prof_begin(h, SOUP_PROF_ES_BUILD);
     fill(h, h->b.head_of_slot, SOUP_NONE, n);
     fill(h, h->b.head_flag, 0u, n);
     hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
@@ -203,22 +203,19 @@ static void ph_rebuild(SoupHandle* h) {
     hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
     prof_end(h, SOUP_PROF_ES_BUILD, e1);
   }
-  hipEvent_t e2 = prof_begin(h);
+  hipEvent_t e2 = prof_begin(h, SOUP_PROF_VERLET_BUILD);
   hipLaunchKernelGGL(k_verlet_build, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_VERLET_BUILD, e2);
   fill(h, h->b.flags + 2, 0u, 1);
 }
 
-// list force + far DH
+// list force + far DH (one kernel: heads add their far row in the same pass)
 static void ph_force(SoupHandle* h) {
-  hipEvent_t e = prof_begin(h);
-  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_FORCE);
+  if (es_on(h)) hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b,
+                     es_on(h) ? 1 : 0);
   prof_end(h, SOUP_PROF_FORCE, e);
-  if (es_on(h)) {
-    hipEvent_t e1 = prof_begin(h);
-    hipLaunchKernelGGL(k_es_far, dim3(xcd_blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
-    prof_end(h, SOUP_PROF_ES_FAR, e1);
-  }
 }
 
 static void ph_force_brute(SoupHandle* h) {
@@ -229,7 +226,7 @@ static void ph_force_brute(SoupHandle* h) {
 // encodeBondChemistry (soup/src/soup-bond-chemistry.ts:24-40): formation, then breaking
 static void ph_bonds(SoupHandle* h) {
   const uint32_t n = h->n;
-  hipEvent_t e = prof_begin(h);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_BONDS);
   fill(h, h->b.flag_cc, 0u, n);
   fill(h, h->b.ev_flag, 0u, n);
   hipLaunchKernelGGL(k_bond_propose, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -325,7 +322,7 @@ static int sweep(SoupHandle* h, SoupSweepResult* out) {
   fill(h, h->pstats, 0u, 4);
   HC(h, hipMemcpyAsync(h->pstats + 4, &h->prng, 4, hipMemcpyHostToDevice, h->stream));
   size_t lds = (size_t)h->nh * sizeof(float);
-  hipEvent_t e = prof_begin(h);
+  hipEvent_t e = prof_begin(h, SOUP_PROF_PROTON);
   hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, h->ptab,
                      h->ptab_g, h->pstats + 4, h->pstats, h->pdstats);
   prof_end(h, SOUP_PROF_PROTON, e);
@@ -480,7 +477,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.head_flag, n);
         DA(h, b.head_pos, (size_t)n + 1);
         DA(h, b.head_rows, nh);
-        DA(h, b.head_xyz, nh);
+        DA(h, b.head_q, nh);
         if (h->nh > 0) {
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
@@ -505,7 +502,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_flag = b.head_flag;
         a.head_pos = b.head_pos;
         a.head_rows = b.head_rows;
-        a.head_xyz = b.head_xyz;
+        a.head_q = b.head_q;
         // initial state: slot = particle index
         std::vector<uint4> bs(n);
         for (uint32_t i = 0; i < n; ++i)
@@ -556,7 +553,7 @@ int soup_profile(SoupHandle* h, int enable) {
   if (!h) return SOUP_E_INVALID;
   int r = sync(h);
   if (r) return r;
-  h->prof.on = enable != 0;
+  h->prof.mask = enable < 0 ? (1u << SOUP_PROF_PHASES) - 1u : (uint32_t)enable;
   for (int k = 0; k < SOUP_PROF_PHASES; ++k) {
     h->prof.ms[k] = 0.0;
     h->prof.count[k] = 0;
@@ -673,7 +670,7 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
       break;
     case SOUP_FIELD_ES_LIST:
       hipLaunchKernelGGL(k_es_to_index, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b.es, h->b.es_count,
-                         h->b.id, reinterpret_cast<uint32_t*>(h->stage), h->nh, h->c.es_cap);
+                         h->b.head_id, reinterpret_cast<uint32_t*>(h->stage), h->nh, h->c.es_cap);
       break;
     case SOUP_FIELD_ES_COUNT: src = h->b.es_count; break;
     case SOUP_FIELD_HEAD_IDX: src = h->b.head_id; break;

@@ -410,8 +410,17 @@ class SoupSystem:
                 "maxForceBefore": before[0], "maxForceAfter": after[0], "nonFiniteBefore": before[1], "nonFiniteAfter": after[1]}
 
     # -- profiling -----------------------------------------------------------------------------------
-    def profile(self, enable: bool) -> None:
-        self._call(self._lib.soup_profile, C.c_int(1 if enable else 0))
+    def profile(self, enable: bool, phases=None) -> None:
+        """Starts (resets) or stops per-phase device timing; ``phases`` limits it to the named phases."""
+        if not enable:
+            mask = 0
+        elif phases is None:
+            mask = -1
+        else:
+            mask = 0
+            for ph in phases:
+                mask |= 1 << native.PROF_PHASES.index(ph)
+        self._call(self._lib.soup_profile, C.c_int(mask))
 
     def profileRead(self) -> Dict[str, Dict[str, float]]:
         ms = (C.c_double * 16)()
