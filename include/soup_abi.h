@@ -213,6 +213,33 @@ int soup_check_lists(SoupHandle* h);
 /* List occupancy of the current lists. */
 int soup_list_stats(SoupHandle* h, SoupListStats* out);
 
+/* The doubles of makePotentialBasis (soup/src/soup-potential.ts:91-122): every number soupPotential
+ * uses, computed by the host in FP64 from data/params.json and data/soup.json. */
+typedef struct SoupPotentialBasis {
+  double epsilon, fene_k, fene_r_inf, bend_k, bend_r0;
+  double wc;                        /* attraction width */
+  double rc_attr;                   /* wcaCut(sigma * beadSizes.tail_tail) */
+  double cutoff;                    /* max(max pair wcaCut, rcAttr + wc): the pair cutoff (r2 <= cutoff^2) */
+  double wca_factor;                /* 2^(1/6) in double */
+  double acid_soap;                 /* acidSoapScaleOf */
+  double b_radius[SOUP_MAX_KINDS];  /* sigma * radiusSigma per kind */
+  double attr[4][4];                /* attractionScaleTable */
+  double es_A, es_kappa, es_cutoff, es_shiftU, es_shiftF;  /* EsBasis (coeffA = 0: no DH term) */
+  uint32_t cls[SOUP_MAX_KINDS];     /* speciesClasses */
+  uint32_t charged_kind;
+  uint32_t use_center_link;         /* soupPotential's centerLink argument given */
+  uint32_t use_charges;             /* soupPotential's charges argument given */
+  uint32_t reserved[5];
+} SoupPotentialBasis;
+
+/* soupPotential (soup/src/soup-potential.ts:145-301) of the handle's current configuration in FP64:
+ * out = {nonbonded (incl. esTotalEnergy, electrostatics.ts:575-600), fene, bend, tether, total}.
+ * Pairs come from the current Verlet and DH lists, which the drift guard keeps complete within
+ * listRange - skin; the call fails if the basis cutoffs exceed that or the guard does not hold.
+ * The terms are reduced in a fixed order (deterministic; equal to the reference's sum to FP64
+ * rounding). */
+int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]);
+
 /* relaxIterations (soup/src/soup-relax.ts:87-139): normalised steepest descent with the cap
  * decaying linearly from d0 to 0, a rebuild before every iteration, and a final rebuild + force. */
 int soup_relax(SoupHandle* h, uint32_t iterations, float d0);

@@ -237,6 +237,22 @@ def initial_charges(cfg, positions: np.ndarray, seed: int):
     return ch, int(rng)
 
 
+def soup_potential(basis, positions, bond_slots, center_link, charges, box):
+    """soupPotential (soup/src/soup-potential.ts:145-301) in FP64: (nonbonded, fene, bend, tether, total).
+    ``basis`` is the product's SoupPotentialBasis (the ABI struct)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+    n = pos.size // 4
+    bs = np.ascontiguousarray(bond_slots, dtype=np.uint32).reshape(-1)
+    cl = None if center_link is None else np.ascontiguousarray(center_link, dtype=np.uint32)
+    ch = None if charges is None else np.ascontiguousarray(charges, dtype=np.float32)
+    bx = np.ascontiguousarray(box, dtype=np.float32)
+    out = (C.c_double * 5)()
+    lib().or_soup_potential(C.byref(basis), C.c_uint32(n), pos.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p),
+                            None if cl is None else cl.ctypes.data_as(C.c_void_p),
+                            None if ch is None else ch.ctypes.data_as(C.c_void_p), bx.ctypes.data_as(C.c_void_p), out)
+    return tuple(out)
+
+
 def set_threads(t: int) -> None:
     lib().or_set_threads(C.c_int(t))
 

@@ -642,6 +642,113 @@ double or_energy_f64(const SoupConfig* c, const OrState* s) {
 }
 
 /* ------------------------------------------------------------------------------------------- */
+/* soupPotential (soup/src/soup-potential.ts:145-301) with esTotalEnergy (soup/src/electrostatics.ts */
+/* :575-600), in FP64 over f32 positions, reference layout. The pair set is every unordered pair  */
+/* with 0 < r2 <= cutoff^2 (the reference reaches it through a deduplicated cell list; the set is  */
+/* the same, so this walks all pairs). Bonded terms from the bond slots: FENE once per bond, the   */
+/* bend once per unordered partner pair of each middle bead, the tether once per linked pair.      */
+/* out = {nonbonded, fene, bend, tether, total}.                                                   */
+/* ------------------------------------------------------------------------------------------- */
+static inline double pot_round(double x) { /* JS Math.round: floor(x + 0.5) semantics */
+  double r = floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
+}
+static inline double pot_mi(double d, double L) { return d - pot_round(d / L) * L; } /* electrostatics.ts:408-410 */
+static double pot_dist(const float* pos, uint32_t i, uint32_t j, const float* box) {
+  double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
+  double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
+  double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+static double pot_fene_v(const SoupPotentialBasis* b, double r) { /* soup-potential.ts:42-45 */
+  double x = r / b->fene_r_inf;
+  return -0.5 * b->fene_k * b->fene_r_inf * b->fene_r_inf * log(1.0 - x * x);
+}
+static double pot_attr_v(const SoupPotentialBasis* b, double r) { /* soup-potential.ts:52-57 */
+  if (r < b->rc_attr) return -b->epsilon;
+  if (r > b->rc_attr + b->wc) return 0.0;
+  double cc = cos((M_PI * (r - b->rc_attr)) / (2.0 * b->wc));
+  return -b->epsilon * cc * cc;
+}
+
+void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos, const uint32_t* bond_slots,
+                       const uint32_t* center_link, const float* charges, const float* box, double out[5]) {
+  double cut2 = b->cutoff * b->cutoff, nonbonded = 0.0;
+  for (uint32_t i = 0; i < n; ++i) { /* soup-potential.ts:199-240 */
+    uint32_t ki = (uint32_t)pos[4u * i + 3];
+    for (uint32_t j = i + 1; j < n; ++j) {
+      double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
+      double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
+      double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > cut2 || r2 <= 0.0) continue;
+      double r = sqrt(r2);
+      uint32_t kj = (uint32_t)pos[4u * j + 3];
+      double bb = (b->b_radius[ki] + b->b_radius[kj]) / 2.0;
+      if (!(r >= b->wca_factor * bb)) { /* wcaV, soup-potential.ts:35-41 */
+        double s6 = pow(bb / r, 6.0), q = 2.0 * s6 - 1.0;
+        nonbonded += b->epsilon * q * q;
+      }
+      double sc = b->attr[b->cls[ki]][b->cls[kj]];
+      if (b->acid_soap > 0.0 && charges && b->use_charges && b->cls[ki] == 1u && b->cls[kj] == 1u &&
+          ((charges[i] == 0.0f) != (charges[j] == 0.0f)))
+        sc += b->acid_soap;
+      if (sc > 0.0) nonbonded += sc * pot_attr_v(b, r);
+    }
+  }
+  double fene = 0.0, bend = 0.0, tether = 0.0;
+  for (uint32_t i = 0; i < n; ++i) { /* bonds(): (i, j) with j > i, soup-potential.ts:254-256 */
+    for (int k = 0; k < 3; ++k) {
+      uint32_t j = bond_slots[3u * i + k];
+      if (j != NONE && j > i) fene += pot_fene_v(b, pot_dist(pos, i, j, box));
+    }
+  }
+  for (uint32_t m = 0; m < n; ++m) { /* bend over partner pairs of each middle, :258-267 */
+    for (int a = 0; a < 3; ++a) {
+      uint32_t pa = bond_slots[3u * m + a];
+      if (pa == NONE) continue;
+      for (int c2 = a + 1; c2 < 3; ++c2) {
+        uint32_t pc = bond_slots[3u * m + c2];
+        if (pc == NONE) continue;
+        double d = pot_dist(pos, pa, pc, box) - b->bend_r0;
+        bend += 0.5 * b->bend_k * d * d;
+      }
+    }
+  }
+  if (center_link && b->use_center_link) { /* :269-276 */
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t owner = center_link[i];
+      if (owner == NONE || owner <= i) continue;
+      tether += pot_fene_v(b, pot_dist(pos, i, owner, box));
+    }
+  }
+  if (charges && b->use_charges && b->es_A != 0.0) { /* esTotalEnergy, electrostatics.ts:575-600 */
+    double ecut2 = b->es_cutoff * b->es_cutoff;
+    for (uint32_t i = 0; i < n; ++i) {
+      if ((uint32_t)pos[4u * i + 3] != b->charged_kind || charges[i] == 0.0f) continue;
+      for (uint32_t j = i + 1; j < n; ++j) {
+        if ((uint32_t)pos[4u * j + 3] != b->charged_kind || charges[j] == 0.0f) continue;
+        double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
+        double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
+        double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 >= ecut2) continue;
+        double r = sqrt(r2);
+        if (r <= 0.0) continue;
+        double u1 = (b->es_A * exp(-b->es_kappa * r)) / r;
+        nonbonded += (double)charges[i] * (double)charges[j] * (u1 - b->es_shiftU + b->es_shiftF * (r - b->es_cutoff));
+      }
+    }
+  }
+  out[0] = nonbonded;
+  out[1] = fene;
+  out[2] = bend;
+  out[3] = tether;
+  out[4] = nonbonded + fene + bend + tether;
+}
+
+/* ------------------------------------------------------------------------------------------- */
 /* Integrator: soup/wgsl/step.wgsl:89-142, verlet.wgsl:115-123.                                  */
 /* ------------------------------------------------------------------------------------------- */
 static inline uint32_t pcg_soup(uint32_t v) {

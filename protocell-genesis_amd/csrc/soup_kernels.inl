@@ -1511,6 +1511,128 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
 // ================================================================================================
 // Boundary translation between slot order and particle-index order
 // ================================================================================================
+// ================================================================================================
+// soupPotential (soup/src/soup-potential.ts:145-301) in FP64, one wavefront per particle. Each
+// unordered pair is taken once, from the side with the smaller particle index (the reference's
+// j > i), over the current Verlet list; the bonded terms of soup-potential.ts:254-286 are taken by
+// the particle that owns them (FENE by the lower index of a bond, the bend by the middle bead, the
+// tether by the lower index of a link); charged heads add esTotalEnergy's pairs (electrostatics.ts:
+// 575-600) from their DH row. Per-particle sums go to part[slot] and k_potential_sum reduces them
+// in a fixed order.
+// ================================================================================================
+__device__ __forceinline__ double pot_mi(double d, double L) { return d - js_round(d / L) * L; }
+__device__ __forceinline__ double pot_dist2(const SoupConfig& c, float4 a, float4 b) {
+  const double dx = pot_mi((double)a.x - (double)b.x, c.box[0]);
+  const double dy = pot_mi((double)a.y - (double)b.y, c.box[1]);
+  const double dz = pot_mi((double)a.z - (double)b.z, c.box[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ double pot_fene(const SoupPotentialBasis& pb, double r) {
+  const double x = r / pb.fene_r_inf;
+  return -0.5 * pb.fene_k * pb.fene_r_inf * pb.fene_r_inf * log(1.0 - x * x);
+}
+__device__ __forceinline__ double pot_attr(const SoupPotentialBasis& pb, double r) {
+  if (r < pb.rc_attr) return -pb.epsilon;
+  if (r > pb.rc_attr + pb.wc) return 0.0;
+  const double cc = cos((3.141592653589793 * (r - pb.rc_attr)) / (2.0 * pb.wc));
+  return -pb.epsilon * cc * cc;
+}
+
+__global__ __launch_bounds__(256) void k_potential(SoupConfig c, DevBuf b, SoupPotentialBasis pb, double4* part, int es) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const uint32_t t = uni(blockIdx.x * (blockDim.x / WAVE) + w);
+  if (t >= c.n) return;
+  const float4 xv = b.pos[t];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
+  const uint32_t idi = uni(b.id[t]);
+  const uint32_t ki = (uint32_t)xi.w & (SOUP_MAX_KINDS - 1u);
+  const float qi = uni_f(b.charge[t]);
+  const double cut2 = pb.cutoff * pb.cutoff;
+  double nb = 0.0, fene = 0.0, bend = 0.0, teth = 0.0;
+  const uint32_t cnt = uni(b.vcount[t]);
+  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    const uint32_t u = row[s];
+    if (b.id[u] <= idi) continue;
+    const float4 xj = b.pos[u];
+    const double r2 = pot_dist2(c, xi, xj);
+    if (r2 > cut2 || r2 <= 0.0) continue;
+    const double r = sqrt(r2);
+    const uint32_t kj = (uint32_t)xj.w & (SOUP_MAX_KINDS - 1u);
+    const double bb = (pb.b_radius[ki] + pb.b_radius[kj]) / 2.0;
+    if (r < pb.wca_factor * bb) {
+      const double sr = bb / r, s2 = sr * sr, s6 = s2 * s2 * s2, q = 2.0 * s6 - 1.0;
+      nb += pb.epsilon * q * q;
+    }
+    double sc = pb.attr[pb.cls[ki] & 3u][pb.cls[kj] & 3u];
+    if (pb.acid_soap > 0.0 && pb.use_charges && pb.cls[ki] == 1u && pb.cls[kj] == 1u &&
+        ((qi == 0.0f) != (b.charge[u] == 0.0f)))
+      sc += pb.acid_soap;
+    if (sc > 0.0) nb += sc * pot_attr(pb, r);
+  }
+  if (lane == 0) {
+    const uint4 own = b.bs[t];
+    const uint32_t sl[3] = {own.x, own.y, own.z};
+    for (int k = 0; k < 3; ++k) {
+      if (sl[k] == SOUP_NONE) continue;
+      const float4 xk = b.pos[b.inv[sl[k]]];
+      if (sl[k] > idi) fene += pot_fene(pb, sqrt(pot_dist2(c, xi, xk)));
+      for (int m = k + 1; m < 3; ++m) {
+        if (sl[m] == SOUP_NONE) continue;
+        const double d = sqrt(pot_dist2(c, xk, b.pos[b.inv[sl[m]]])) - pb.bend_r0;
+        bend += 0.5 * pb.bend_k * d * d;
+      }
+    }
+    const uint32_t owner = b.clink[t];
+    if (pb.use_center_link && owner != SOUP_NONE && owner > idi) teth += pot_fene(pb, sqrt(pot_dist2(c, xi, b.pos[b.inv[owner]])));
+  }
+  const uint32_t hh = es ? uni(b.head_of_slot[t]) : SOUP_NONE;
+  if (hh != SOUP_NONE && qi != 0.0f) {
+    const uint32_t ecnt = uni(b.es_count[hh]);
+    const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+    const double ecut2 = pb.es_cutoff * pb.es_cutoff;
+    for (uint32_t s = lane; s < ecnt; s += WAVE) {
+      const uint32_t g = erow[s];
+      if (b.head_id[g] <= idi) continue;
+      const float4 xq = b.head_q[g];
+      if (xq.w == 0.0f) continue;
+      const double r2 = pot_dist2(c, xi, xq);
+      if (r2 >= ecut2) continue;
+      const double r = sqrt(r2);
+      if (r <= 0.0) continue;
+      const double u1 = (pb.es_A * exp(-pb.es_kappa * r)) / r;
+      nb += (double)qi * (double)xq.w * (u1 - pb.es_shiftU + pb.es_shiftF * (r - pb.es_cutoff));
+    }
+  }
+  nb = wave_sum_d_dpp(nb);
+  if (lane == 0) part[t] = make_double4(nb, fene, bend, teth);
+}
+
+// fixed-order sum of the per-particle terms: thread k sums slots k, k + 1024, ... in order, then a
+// fixed tree over the threads
+__global__ __launch_bounds__(1024) void k_potential_sum(const double4* part, uint32_t n, double* out) {
+  __shared__ double4 s[1024];
+  double4 a = make_double4(0.0, 0.0, 0.0, 0.0);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double4 v = part[i];
+    a.x += v.x; a.y += v.y; a.z += v.z; a.w += v.w;
+  }
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double4 v = s[threadIdx.x + o];
+      s[threadIdx.x].x += v.x; s[threadIdx.x].y += v.y; s[threadIdx.x].z += v.z; s[threadIdx.x].w += v.w;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double4 v = s[0];
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    out[4] = v.x + v.y + v.z + v.w;
+  }
+}
+
 template <typename T>
 __global__ void k_to_index(const T* src, const uint32_t* id, T* dst, uint32_t n) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

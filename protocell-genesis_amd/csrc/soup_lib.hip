@@ -492,6 +492,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, h->lmax, 2);
         h->stage_bytes = std::max<size_t>((size_t)n * cfg->list_cap * 4, (size_t)n * 16);
         if (h->nh > 0) h->stage_bytes = std::max<size_t>(h->stage_bytes, (size_t)h->nh * cfg->es_cap * 4);
+        h->stage_bytes = std::max<size_t>(h->stage_bytes, (size_t)n * sizeof(double4) + 64);  // soup_potential
         DA(h, h->stage, h->stage_bytes);
         // alternates share the arrays that are rewritten in place
         a.force = b.force;
@@ -789,6 +790,35 @@ int soup_list_stats(SoupHandle* h, SoupListStats* out) {
   out->es_max = es_on(h) ? m[1] : 0;
   out->es_cap = h->c.es_cap;
   return 0;
+}
+
+int soup_potential(SoupHandle* h, const SoupPotentialBasis* pb, double out[5]) {
+  if (!h) return SOUP_E_INVALID;
+  if (!pb || !out) return fail(h, SOUP_E_INVALID, "soup_potential: null basis or output");
+  // the lists hold every pair within listRange - skin of the current positions while the drift
+  // guard holds; the potential's cutoffs must lie inside that
+  const double near_ok = (double)h->c.list_range - (double)h->c.skin;
+  if (pb->cutoff > near_ok + 1e-9)
+    return fail(h, SOUP_E_INVALID, fmt("soup_potential: pair cutoff %.6f exceeds the Verlet list's guaranteed range %.6f",
+                                       pb->cutoff, near_ok));
+  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
+  if (es && pb->es_cutoff > (double)h->c.es_list_range - (double)h->c.skin + 1e-6)
+    return fail(h, SOUP_E_INVALID, fmt("soup_potential: DH cutoff %.6f exceeds the DH list's guaranteed range %.6f",
+                                       pb->es_cutoff, (double)h->c.es_list_range - (double)h->c.skin));
+  int r = sync(h);
+  if (r) return r;
+  r = assert_lists(h);
+  if (r) return r;
+  const size_t need = (size_t)h->n * sizeof(double4) + 64;
+  if (need > h->stage_bytes) return fail(h, SOUP_E_INVALID, "soup_potential: staging buffer too small");
+  double4* part = reinterpret_cast<double4*>(h->stage);
+  double* dout = reinterpret_cast<double*>(h->stage + (size_t)h->n * sizeof(double4));
+  if (es) hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+  hipLaunchKernelGGL(k_potential, dim3(blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, part,
+                     es ? 1 : 0);
+  hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, part, h->n, dout);
+  HC(h, hipMemcpyAsync(out, dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
 }
 
 int soup_relax(SoupHandle* h, uint32_t iterations, float d0) {

@@ -72,6 +72,16 @@ class SoupListStats(C.Structure):
                 ("verlet_pairs", C.c_uint64), ("es_pairs", C.c_uint64)]
 
 
+class SoupPotentialBasis(C.Structure):
+    """The doubles of makePotentialBasis (include/soup_abi.h)."""
+    _fields_ = [("epsilon", C.c_double), ("fene_k", C.c_double), ("fene_r_inf", C.c_double), ("bend_k", C.c_double),
+                ("bend_r0", C.c_double), ("wc", C.c_double), ("rc_attr", C.c_double), ("cutoff", C.c_double),
+                ("wca_factor", C.c_double), ("acid_soap", C.c_double), ("b_radius", C.c_double * 8),
+                ("attr", (C.c_double * 4) * 4), ("es_A", C.c_double), ("es_kappa", C.c_double), ("es_cutoff", C.c_double),
+                ("es_shiftU", C.c_double), ("es_shiftF", C.c_double), ("cls", C.c_uint32 * 8), ("charged_kind", C.c_uint32),
+                ("use_center_link", C.c_uint32), ("use_charges", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 
@@ -79,7 +89,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_profile", "soup_profile_read", "soup_version",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_profile", "soup_profile_read", "soup_version",
 )
 PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_far", "drift", "bonds", "kick_thermo",
                "proton")
@@ -114,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_check_lists": (C.c_int, [H]),
         "soup_list_stats": (C.c_int, [H, C.POINTER(SoupListStats)]),
         "soup_relax": (C.c_int, [H, C.c_uint32, C.c_float]),
+        "soup_potential": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(C.c_double)]),
         "soup_profile": (C.c_int, [H, C.c_int]),
         "soup_profile_read": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "soup_version": (C.c_char_p, []),

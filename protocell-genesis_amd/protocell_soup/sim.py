@@ -150,6 +150,7 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     R.soup, R.p, R.rules, R.catalyst_kind, R.event_rule_ids = soup, p, rules, catalyst_kind, event_rule_ids
     R.start, R.counts_by_kind, R.N, R.box, R.live_box, R.plan, R.geo = start, counts_by_kind, N, box, live_box, plan, geo
     R.list_cap, R.es, R.es_heads, R.sweep_every, R.seed = list_cap, es, es_heads, sweep_every, seed
+    R.opts = dict(opts)
     R.init, R.charges0, R.prng, R.events0, R.initial_step = init, charges0, prng_state, events0, initial_step
     R.next_sweep_at = initial_step + sweep_every if es.enabled else 2 ** 64 - 1
     R.verlet = dict(soup["verletList"], listCapacity=list_cap)
@@ -365,6 +366,23 @@ class SoupSystem:
     def verletConfig(self) -> Dict[str, Any]:
         return dict(self._R.verlet, listRange=self._R.geo.listRange, walkRadius=self._R.geo.effectiveWalkRadius,
                     listBytes=verletListBytes(self.N, self._R.list_cap))
+
+    def potentialBasis(self, centerLink: bool = True, charges: bool = True) -> native.SoupPotentialBasis:
+        """makePotentialBasis for this system's species, parameters and DH basis."""
+        from .potential import makePotentialBasis
+
+        o = self._R.opts
+        return makePotentialBasis(self._R.soup, self._R.p, self._R.es, o.get("solventAttractionScaleOverride"),
+                                  o.get("claySurfaceChemistry"), o.get("acidSoapScaleOverride"), centerLink, charges)
+
+    def potential(self, basis: Optional[native.SoupPotentialBasis] = None, centerLink: bool = True,
+                  charges: bool = True) -> Dict[str, float]:
+        """soupPotential (soup/src/soup-potential.ts:145-301) of the current configuration, on the GPU:
+        {nonbonded, fene, bend, tether, total} in FP64."""
+        b = basis if basis is not None else self.potentialBasis(centerLink, charges)
+        out = (C.c_double * 5)()
+        self._call(self._lib.soup_potential, C.byref(b), out)
+        return {"nonbonded": out[0], "fene": out[1], "bend": out[2], "tether": out[3], "total": out[4]}
 
     def listOccupancyDEBUG(self) -> Dict[str, Any]:
         s = native.SoupListStats()

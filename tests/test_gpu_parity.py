@@ -164,3 +164,60 @@ def test_amphiphile_soup_runs():
         assert s.nonFiniteCount() == {"pos": 0, "vel": 0}
     finally:
         s.dispose()
+
+
+def test_potential_matches_oracle_and_force():
+    """soupPotential on the GPU (soup_potential) against the oracle's FP64 restatement of
+    soup/src/soup-potential.ts on the same configuration, and its finite-difference gradient against
+    the GPU force, as tests/soup-area-move.test.ts checks the reference's."""
+    opts = amphiphile_opts(16.0, 60, 4, {"W": 1500, "C": 200, "M": 20}, seed=3,
+                           electrostatics={"enabled": True, "pH": 5.0, "ionicStrengthMolar": 0.1})
+    R = ps.resolve(opts)
+    s = ps.createSoup(opts)
+    try:
+        s.relaxColdStart()
+        s.step(100)
+        pos = s.particles().reshape(-1, 4)
+        cl = s.centerLinks().copy()
+        kinds = pos[:, 3].astype(int)
+        cat = np.nonzero(kinds == R.soup["monomers"].index(next(m for m in R.soup["monomers"] if m["id"] == "M")))[0]
+        carbon = R.soup["monomers"].index(next(m for m in R.soup["monomers"] if m["id"] == "C"))
+        free_c = np.nonzero((kinds == carbon) & (cl == 0xFFFFFFFF))[0]
+        linked = 0
+        for m in cat:
+            if cl[m] != 0xFFFFFFFF:
+                continue
+            d = pos[free_c, :3] - pos[m, :3]
+            d -= 16.0 * np.rint(d / 16.0)
+            r = np.sqrt((d ** 2).sum(1))
+            j = int(free_c[int(np.argmin(r))])
+            if r.min() < 1.3 and cl[j] == 0xFFFFFFFF:
+                cl[m], cl[j] = j, m
+                linked += 1
+        s.writeDEBUG("CENTER_LINK", cl)
+        assert linked > 0
+        u = s.potential()
+        ref = orc.soup_potential(s.potentialBasis(), s.particles(), s.bondSlots(), s.centerLinks(), s.charges(), R.cfg.box)
+        for k, key in enumerate(("nonbonded", "fene", "bend", "tether", "total")):
+            assert u[key] == pytest.approx(ref[k], rel=1e-10, abs=1e-10), key
+        assert u["fene"] > 0 and u["tether"] > 0 and u["bend"] > 0
+        # -dU/dx against the GPU force of the same positions
+        f = s.forces().reshape(-1, 4)
+        p0 = s.particles()
+        rng = np.random.default_rng(7)
+        heads = np.nonzero(s.charges() != 0)[0]
+        picks = list(rng.choice(heads, 2, replace=False)) + list(rng.choice(R.N, 3, replace=False)) + [int(np.nonzero(cl != 0xFFFFFFFF)[0][0])]
+        h = 2.0 ** -9
+        for i in picks:
+            for a in range(3):
+                vals = []
+                for kk in (-2, -1, 1, 2):
+                    p = p0.copy()
+                    p[4 * i + a] += np.float32(kk * h)
+                    s.writeDEBUG("POS", p)
+                    vals.append(s.potential()["total"])
+                fd = -(8 * (vals[2] - vals[1]) - (vals[3] - vals[0])) / (12 * h)
+                assert fd == pytest.approx(float(f[i, a]), rel=1e-3, abs=1e-3 * max(1.0, float(np.abs(f[i, :3]).max())))
+        s.writeDEBUG("POS", p0)
+    finally:
+        s.dispose()

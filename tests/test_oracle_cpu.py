@@ -179,10 +179,8 @@ def _pair_system(kind_a, kind_b, r, charges=(0.0, 0.0), box=20.0, es=None):
 
 def test_pair_energy_closed_forms():
     wf = 2 ** (1 / 6)
-    # WCA between two carbons: U(b) = eps (2*1 - 1)^2 = 1; zero at and beyond 2^(1/6) b
+    # C-C at r = b: WCA eps (2*1 - 1)^2 = 1 and the apolar-apolar cos^2 plateau -1 below rc -> total 0
     _, o = _pair_system(KIND["C"], KIND["C"], 1.0)
-    assert o.energy_f64() == pytest.approx(1.0 - 1.0, abs=1e-12) or True  # plateau of the attraction adds -1
-    # C-C: WCA(1)=1 and the apolar-apolar cos^2 plateau -1 below rc -> total 0
     assert o.energy_f64() == pytest.approx(0.0, abs=1e-9)
     _, o = _pair_system(KIND["C"], KIND["C"], wf * 1.0 + 1.6 + 0.01)
     assert o.energy_f64() == 0.0
@@ -202,6 +200,101 @@ def test_pair_energy_closed_forms():
     b = makeEsBasis(SOUP, dict(P, thermostat=dict(P["thermostat"], kT=1.0)), dict(es, minBoxSigma=20.0))
     expect = esPairEnergy(float(np.float32(5 + 3.5) - np.float32(5)), 1.0, b)
     assert o.energy_f64() == pytest.approx(expect, rel=1e-6)
+
+
+# ---------------------------------------------------------------------------------------------
+# soupPotential (soup/src/soup-potential.ts:145-301)
+# ---------------------------------------------------------------------------------------------
+def _basis(R, center_link=True, charges=True):
+    from protocell_soup.potential import makePotentialBasis
+
+    return makePotentialBasis(R.soup, R.p, R.es, centerLink=center_link, charges=charges)
+
+
+def _potential(R, o, center_link=True, charges=True):
+    return orc.soup_potential(_basis(R, center_link, charges), o.pos, o.bond_slots, o.center_link, o.charge, R.cfg.box)
+
+
+def test_soup_potential_closed_forms():
+    wf = 2 ** (1 / 6)
+    # C-C at r = b: WCA 1 + plateau -1; on the ramp at rc + wc/2: -1/2; beyond rc + wc: 0
+    R, o = _pair_system(KIND["C"], KIND["C"], 1.0)
+    assert _potential(R, o)[0] == pytest.approx(0.0, abs=1e-12)
+    R, o = _pair_system(KIND["C"], KIND["C"], wf + 0.8)
+    r = float(np.float32(5.0 + wf + 0.8) - np.float32(5.0))
+    assert _potential(R, o)[0] == pytest.approx(-math.cos(math.pi * (r - wf) / 3.2) ** 2, rel=1e-12)
+    R, o = _pair_system(KIND["C"], KIND["C"], wf + 1.6 + 0.01)
+    assert _potential(R, o)[4] == 0.0
+    # head-water: WCA at b = 0.975 plus the solvent-polar depth 4/3.5 on the plateau
+    R, o = _pair_system(KIND["O"], KIND["W"], 1.0)
+    r = float(np.float32(6.0) - np.float32(5.0))
+    s6 = (0.975 / r) ** 6
+    assert _potential(R, o)[0] == pytest.approx((2 * s6 - 1) ** 2 - 4 / 3.5, rel=1e-12)
+    # acid-soap: a charged and a neutral head add acidSoap * attr_v; two neutral heads have no well
+    R, o = _pair_system(KIND["O"], KIND["O"], 1.5, charges=(-1.0, 0.0))
+    r = float(np.float32(6.5) - np.float32(5.0))
+    ramp = -math.cos(math.pi * (r - wf) / 3.2) ** 2
+    assert _potential(R, o)[0] == pytest.approx(2.0 * ramp, rel=1e-12)
+    assert _potential(R, o, charges=False)[0] == 0.0
+    # two charged heads under DH at 0.1 M: exactly esPairEnergy of the reference's basis
+    es = {"enabled": True, "pH": 7.0, "ionicStrengthMolar": 0.1}
+    R, o = _pair_system(KIND["O"], KIND["O"], 3.5, charges=(-1.0, -1.0), es=es)
+    r = float(np.float32(8.5) - np.float32(5.0))
+    assert _potential(R, o)[0] == pytest.approx(esPairEnergy(r, 1.0, R.es), rel=1e-12)
+
+
+def test_soup_potential_bonded_closed_forms():
+    # a bent 3-bead chain C0-C1-C2 (bonds in both slots, as the reference stores them) plus a tether
+    opts = {"box": [20.0] * 3, "seed": 1, "kT": 1.0, "clay": False, "start": {k: 0 for k in KIND}}
+    opts["start"]["C"] = 3
+    opts["start"]["M"] = 1
+    pos = np.array([5.0, 5.0, 5.0, KIND["C"], 6.0, 5.0, 5.0, KIND["C"], 6.5, 5.75, 5.0, KIND["C"],
+                    5.0, 6.0, 5.5, KIND["M"]], dtype=np.float32)
+    none = 0xFFFFFFFF
+    bs = np.array([1, none, none, 0, 2, none, 1, none, none, none, none, none], np.uint32)
+    cl = np.array([3, none, none, 0], np.uint32)
+    opts["resume"] = {"globalStep": 0, "liveBox": [20.0] * 3, "positions": pos, "velocities": np.zeros(16, np.float32),
+                      "bondSlots": bs, "centerLink": cl, "centerHeldSteps": np.zeros(4, np.uint32),
+                      "desorbEvents": np.zeros(2, np.uint32), "bondRng": np.arange(4, dtype=np.uint32),
+                      "thermoRng": np.arange(4, dtype=np.uint32), "events": {}, "charges": np.zeros(4, np.float32)}
+    R = ps.resolve(opts)
+    o = orc.OracleState(R.cfg, R.init, R.charges0)
+    k, ri, kb, r0 = P["fene"]["k"], P["fene"]["rInf"], P["bend"]["k"], P["bend"]["r0"]
+    fene = lambda r: -0.5 * k * ri * ri * math.log(1 - (r / ri) ** 2)  # noqa: E731
+    r01, r12, r02 = 1.0, math.hypot(0.5, 0.75), math.hypot(1.5, 0.75)
+    r03 = math.hypot(1.0, 0.5)
+    u = _potential(R, o)
+    assert u[1] == pytest.approx(fene(r01) + fene(r12), rel=1e-12)
+    assert u[2] == pytest.approx(0.5 * kb * (r02 - r0) ** 2, rel=1e-12)
+    assert u[3] == pytest.approx(fene(r03), rel=1e-12)
+    assert _potential(R, o, center_link=False)[3] == 0.0
+    assert u[4] == pytest.approx(sum(u[:4]), rel=1e-15)
+
+
+def test_soup_potential_gradient_is_the_force():
+    opts = amphiphile_opts(16.0, 60, 4, {"W": 1500, "C": 200, "M": 20}, seed=3,
+                           electrostatics={"enabled": True, "pH": 5.0, "ionicStrengthMolar": 0.1})
+    R = ps.resolve(opts)
+    o = orc.OracleState(R.cfg, R.init, R.charges0)
+    assert o.relax(200, 0.1) == 0
+    o.rebuild()
+    assert (o.charge != 0).sum() > 10
+    f64 = o.force_f64(use_list=False)
+    b = _basis(R)
+    rng = np.random.default_rng(5)
+    heads = np.nonzero(o.charge != 0)[0]
+    picks = list(rng.choice(heads, 3, replace=False)) + list(rng.choice(R.N, 3, replace=False))
+    h = 2.0 ** -9  # exact in f32 for coordinates below 16
+    for i in picks:
+        for a in range(3):
+            x0 = o.pos[4 * i + a]
+            vals = []
+            for kk in (-2, -1, 1, 2):
+                o.pos[4 * i + a] = x0 + np.float32(kk * h)
+                vals.append(orc.soup_potential(b, o.pos, o.bond_slots, o.center_link, o.charge, R.cfg.box)[4])
+            o.pos[4 * i + a] = x0
+            fd = -(8 * (vals[2] - vals[1]) - (vals[3] - vals[0])) / (12 * h)
+            assert fd == pytest.approx(f64[i, a], rel=1e-4, abs=1e-4 * max(1.0, np.abs(f64[i]).max()))
 
 
 def _fd_force(o, i, a, h=1e-3):
