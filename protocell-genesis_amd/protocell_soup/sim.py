@@ -268,6 +268,15 @@ class SoupSystem:
     def box(self):
         return list(self._R.live_box)
 
+    @property
+    def cyclePhase(self) -> str:
+        """'none': dry-wet cycling is not part of this path (soup/src/soup-types.ts cyclePhase)."""
+        return "none"
+
+    @property
+    def cycleIndex(self) -> int:
+        return 0
+
     # -- readbacks (soup/src/soup-readback.ts) ----------------------------------------------------
     def particles(self) -> np.ndarray:
         return self._read("POS", np.float32)
