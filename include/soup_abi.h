@@ -240,6 +240,60 @@ typedef struct SoupPotentialBasis {
  * rounding). */
 int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]);
 
+/* ---------------------------------------------------------------------------------------------
+ * Snapshot analysis (soup/src/stages.ts:204-289's per-sample work): an analysis context holds one
+ * particle snapshot in device memory -- copied from a live system or from host arrays -- and answers
+ * the grid and graph questions of the reference's CPU analysis with the same results:
+ * findAmphiphiles' components (soup/src/amphiphile.ts:55-141), clusterComponents' labels
+ * (engine/src/aggregate.ts:65-212), encapsulatedWaterVolume's flood (soup/src/water-closure.ts:
+ * 119-363) and localCavityVolume's open-boundary flood (soup/src/aggregates.ts:346-383,
+ * engine/src/closure.ts:58-170). Each context has its own HIP stream; calls on one context must not
+ * overlap.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct SoupAnalysis SoupAnalysis;
+
+/* Cell and water counts of one periodic encapsulated-water flood. */
+typedef struct SoupWaterCounts {
+  uint64_t total_empty_cells;    /* cells not within radius of a member */
+  uint64_t unreached_cells;      /* empty cells the flood from the seed never reaches */
+  uint64_t reached_empty_cells;  /* empty cells it reaches */
+  uint32_t total_water;          /* beads of the water kind */
+  uint32_t encapsulated;         /* water beads in empty, unreached cells */
+  uint32_t reached_empty_water;  /* water beads in empty, reached cells */
+  uint32_t seed_cell;            /* farthestEmptyCellSeed, x-fastest cell index */
+  uint32_t dims[3];              /* ceil(box / cell) per axis */
+  uint32_t reserved;
+} SoupWaterCounts;
+
+int soup_analysis_create(SoupAnalysis** out);
+int soup_analysis_destroy(SoupAnalysis* a);
+const char* soup_analysis_last_error(const SoupAnalysis* a);
+/* The snapshot: n particles (positions x, y, z, kind as 4 floats; bond slots 3 u32 each) from host
+ * memory, or the current state of a live system (particle-index order, device to device). */
+int soup_analysis_load(SoupAnalysis* a, uint32_t n, const float* pos4, const uint32_t* bond_slots);
+int soup_analysis_load_system(SoupAnalysis* a, SoupHandle* h);
+/* findAmphiphiles (soup/src/amphiphile.ts:55-141): bit k of polar_mask = kind k is polar. Counts
+ * the amphiphiles and their chain beads; soup_analysis_amphiphiles_read then returns, in the
+ * reference's result order, each head, chain_offsets (n_amph + 1) into chains (tail 1 walked out
+ * from the head, then tail 2) and tail_lengths (2 per amphiphile, 0 for an absent second tail). */
+int soup_analysis_amphiphiles(SoupAnalysis* a, uint32_t polar_mask, uint32_t* n_amph, uint32_t* n_chain);
+int soup_analysis_amphiphiles_read(SoupAnalysis* a, uint32_t* heads, uint32_t* chain_offsets, uint32_t* chains,
+                                   uint32_t* tail_lengths);
+/* clusterComponents' connectivity over members[0..m) (particle indices, in the caller's local
+ * order): x and y periodic, z periodic when periodic_z (the soup's convention) or open (the membrane
+ * convention of largestClusterCenter, positions then in [0, box z)); labels[k] = the smallest
+ * local index of k's component. */
+int soup_analysis_clusters(SoupAnalysis* a, const uint32_t* members, uint32_t m, const double box[3], double cutoff,
+                           int periodic_z, uint32_t* labels);
+/* encapsulatedWaterVolume's occupancy (members within radius, periodic), farthest empty seed from
+ * centre, 6-connected periodic flood, and the water-bead counts of kind water_kind. */
+int soup_analysis_water(SoupAnalysis* a, const uint32_t* members, uint32_t m, uint32_t water_kind, const double box[3],
+                        const double centre[3], double cell, double radius, SoupWaterCounts* out);
+/* occupancy + enclosedVolume over m given positions (x, y periodic when marking, z open; flood from
+ * the six faces): the number of empty cells the flood never reaches. */
+int soup_analysis_cavity(SoupAnalysis* a, const float* pos4, uint32_t m, const double box[3], double cell, double radius,
+                         uint64_t* unreached_cells);
+
 /* relaxIterations (soup/src/soup-relax.ts:87-139): normalised steepest descent with the cap
  * decaying linearly from d0 to 0, a rebuild before every iteration, and a final rebuild + force. */
 int soup_relax(SoupHandle* h, uint32_t iterations, float d0);

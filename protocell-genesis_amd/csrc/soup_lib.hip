@@ -143,14 +143,15 @@ static void fill(SoupHandle* h, uint32_t* p, uint32_t v, uint32_t count) {
   if (count) hipLaunchKernelGGL(k_fill_u32, dim3(blocks(count, 256)), dim3(256), 0, h->stream, p, v, count);
 }
 
-// exclusive scan of m u32 -> out[0..m]
-static void scan(SoupHandle* h, const uint32_t* in, uint32_t m, uint32_t* out) {
+// exclusive scan of m u32 -> out[0..m]; block_sums holds SCAN_TILE + 1 entries
+static void scan_on(hipStream_t st, uint32_t* block_sums, const uint32_t* in, uint32_t m, uint32_t* out) {
   uint32_t nb = blocks(m, SCAN_TILE);
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, m, h->b.scan_blocks);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, h->stream, h->b.scan_blocks, nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, h->stream, in, m, h->b.scan_blocks, nb, out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, m, block_sums);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_BLOCK), 0, st, block_sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, m, block_sums, nb, out);
 }
+static void scan(SoupHandle* h, const uint32_t* in, uint32_t m, uint32_t* out) { scan_on(h->stream, h->b.scan_blocks, in, m, out); }
 
 static bool es_on(const SoupHandle* h) { return h->c.es_enabled && h->nh > 0; }
 
@@ -843,3 +844,5 @@ int soup_relax(SoupHandle* h, uint32_t iterations, float d0) {
 }
 
 }  // extern "C"
+
+#include "soup_analysis.inl"

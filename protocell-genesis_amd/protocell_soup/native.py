@@ -82,6 +82,12 @@ class SoupPotentialBasis(C.Structure):
                 ("use_center_link", C.c_uint32), ("use_charges", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
+class SoupWaterCounts(C.Structure):
+    _fields_ = [("total_empty_cells", C.c_uint64), ("unreached_cells", C.c_uint64), ("reached_empty_cells", C.c_uint64),
+                ("total_water", C.c_uint32), ("encapsulated", C.c_uint32), ("reached_empty_water", C.c_uint32),
+                ("seed_cell", C.c_uint32), ("dims", C.c_uint32 * 3), ("reserved", C.c_uint32)]
+
+
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 
@@ -89,7 +95,9 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_profile", "soup_profile_read", "soup_version",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
+    "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
+    "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_version",
 )
 PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_far", "drift", "bonds", "kick_thermo",
                "proton")
@@ -125,6 +133,18 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_list_stats": (C.c_int, [H, C.POINTER(SoupListStats)]),
         "soup_relax": (C.c_int, [H, C.c_uint32, C.c_float]),
         "soup_potential": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(C.c_double)]),
+        "soup_analysis_create": (C.c_int, [C.POINTER(H)]),
+        "soup_analysis_destroy": (C.c_int, [H]),
+        "soup_analysis_last_error": (C.c_char_p, [H]),
+        "soup_analysis_load": (C.c_int, [H, C.c_uint32, C.c_void_p, C.c_void_p]),
+        "soup_analysis_load_system": (C.c_int, [H, H]),
+        "soup_analysis_amphiphiles": (C.c_int, [H, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "soup_analysis_amphiphiles_read": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "soup_analysis_clusters": (C.c_int, [H, C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_void_p]),
+        "soup_analysis_water": (C.c_int, [H, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.c_double, C.c_double, C.POINTER(SoupWaterCounts)]),
+        "soup_analysis_cavity": (C.c_int, [H, C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_double,
+                                           C.POINTER(C.c_uint64)]),
         "soup_profile": (C.c_int, [H, C.c_int]),
         "soup_profile_read": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "soup_version": (C.c_char_p, []),
