@@ -6,7 +6,8 @@ Workload (the campaign-scale configuration): a 76 sigma periodic box with 483,26
 rebuild every 10 steps, constant-pH sweep every 1000 steps, seeded jittered-lattice start followed by
 the cold-start minimiser. Inputs are resident in HBM before the timed region; the timed region is
 exactly --steps integration steps of SoupSystem.step (with the per-1000-step guards and the sweeps
-that fall inside it).
+that fall inside it) and, every 1000 steps, the snapshot analysis of the campaign (amphiphile
+recognition, aggregate decomposition, encapsulated-water flood: detectStage).
 
 Prints ONE JSON line: steps/s, the roofline of the dominant kernel (the list force kernel, which
 also sums the far DH rows of charged heads; algorithmic bytes per launch = N * (56 + 4 * mean Verlet
@@ -34,6 +35,7 @@ CAMPAIGN = {
     "electrostatics": {"enabled": True, "pH": 7.0, "ionicStrengthMolar": 0.01},
 }
 HBM_PEAK_GBS = 8000.0
+ANALYSIS_EVERY = 1000  # the campaign's aggregate decomposition + encapsulated-water flood cadence
 REFERENCE_STEPS_PER_S = 11.5  # 87.2 ms/step at 483,268 beads (reference closeout report)
 
 
@@ -42,9 +44,11 @@ def log(msg):
 
 
 def cpu_baseline(sys_, R, sample_steps):
-    """The oracle (C, OpenMP) stepping the same state on the host cores."""
+    """The oracle (C, OpenMP) stepping the same state on the host cores, plus one oracle snapshot
+    analysis, both amortised to the campaign's cadence."""
     import numpy as np
 
+    from oracle import analysis as ora  # checker / baseline only
     from oracle import oracle as orc  # checker / baseline only
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -64,11 +68,36 @@ def cpu_baseline(sys_, R, sample_steps):
     t0 = time.perf_counter()
     o.protonation_sweep()
     t_sweep = time.perf_counter() - t0
-    per_step = t_steps / sample_steps + t_sweep / R.sweep_every
+    t0 = time.perf_counter()
+    _oracle_snapshot_analysis(ora, sys_, R)
+    t_an = time.perf_counter() - t0
+    per_step = t_steps / sample_steps + t_sweep / R.sweep_every + t_an / ANALYSIS_EVERY
     return {"value": 1.0 / per_step, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{sample_steps} oracle steps (incl. {sample_steps // 10} rebuilds, {sample_steps // 20} bond MC) "
-                      f"+ 1 constant-pH sweep amortised over {R.sweep_every} steps, same 483,268-bead state; "
-                      f"{t_steps:.1f} s + {t_sweep:.1f} s"}
+                      f"+ 1 constant-pH sweep amortised over {R.sweep_every} steps + 1 snapshot analysis amortised over "
+                      f"{ANALYSIS_EVERY} steps, same 483,268-bead state; {t_steps:.1f} s + {t_sweep:.1f} s + {t_an:.1f} s"}
+
+
+def _oracle_snapshot_analysis(ora, sys_, R):
+    """The oracle's detectStage work on one snapshot: recogniser, clustering, per-aggregate geometry
+    and floods, box-wide cavity and head peaks."""
+    import numpy as np
+
+    from protocell_soup.analysis import loadStageThresholds, memberCutoff
+
+    t = loadStageThresholds()
+    pos = sys_.particles()
+    box = list(sys_.box)
+    monomers = R.soup["monomers"]
+    amph = ora.find_amphiphiles(pos, sys_.bonds(), monomers)
+    mem = ora.member_indices(amph)
+    water_kind = next(k for k, m in enumerate(monomers) if m["id"] == R.soup["solvent"]["waterId"])
+    water = list(np.nonzero(np.rint(pos.reshape(-1, 4)[:, 3]) == water_kind)[0])
+    ora.analyze_aggregates(pos, box, monomers, amph, mem, memberCutoff(), t, water)
+    if mem:
+        ora.enclosed_volume_box(pos, mem, box, t["closureCell"], t["closureRadius"])
+    ora.head_peaks(pos, box, monomers, t["headDensityBins"], t["minHeadsPerBin"], t["headPeakSeparationMin"],
+                   t["headPeakSeparationMax"])
 
 
 def main():
@@ -121,14 +150,23 @@ def main():
 
     # timed region: HIP events bracket only the force kernel (the roofline's dominant kernel), on the
     # library's own stream
+    from protocell_soup.analysis import AnalysisContext, detectStage
+
+    actx = AnalysisContext()
     s.profile(True, phases=["force"])
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     done = 0
+    t_analysis, n_analysis, stage = 0.0, 0, None
     while done < args.steps:
-        k = min(1000, args.steps - done)
+        k = min(ANALYSIS_EVERY - s.steps % ANALYSIS_EVERY, args.steps - done)
         s.step(k)
         done += k
+        if s.steps % ANALYSIS_EVERY == 0:
+            ta = time.perf_counter()
+            stage, evidence = detectStage(s, actx)
+            t_analysis += time.perf_counter() - ta
+            n_analysis += 1
         log(f"timed {done}/{args.steps}")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
@@ -180,12 +218,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "campaign soup 76 sigma, 483,268 beads, pH 7.0, 0.01 M, long-range DH, bond MC/20, rebuild/10, pH sweep/1000",
+            "workload": "campaign soup 76 sigma, 483,268 beads, pH 7.0, 0.01 M, long-range DH, bond MC/20, rebuild/10, "
+                        "pH sweep/1000, aggregate decomposition + encapsulated-water flood/1000",
             "N": R.N, "box": 76, "mean_verlet": round(lbar, 2), "mean_dh_list": round(occ["esMean"], 2),
             "pair_interactions_per_s": round(pair_per_step * steps_per_s, 1),
             "algorithmic_bytes_per_step": round(bytes_step),
             "step_roofline_frac": round(bytes_step * steps_per_s / (HBM_PEAK_GBS * 1e9), 4),
             "phase_ms_per_step": {k: round(v["ms"] / args.phase_steps, 4) for k, v in phases.items() if v["count"]},
+            "analysis_every": ANALYSIS_EVERY,
+            "analysis_ms_per_sample": round(1000.0 * t_analysis / max(1, n_analysis), 2),
+            "analysis_samples": n_analysis,
+            "last_stage": stage,
         },
         "roofline": {
             "kernel": "k_force_list",
@@ -200,6 +243,7 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    actx.close()
     s.dispose()
     print(json.dumps(out), flush=True)
 
