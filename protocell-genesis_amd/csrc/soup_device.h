@@ -56,8 +56,9 @@ struct DevBuf {
   uint32_t* cell_tmp;
   uint32_t* perm;     // new slot -> old slot
   // lists
-  uint32_t* vl;
+  uint32_t* vl;       // by slot: row of cap entries, near part from the front, far part from the back
   uint32_t* vcount;
+  uint32_t* vnear;    // by slot: entries of the near part (list_split)
   uint32_t* head_id;
   uint32_t* head_slot;
   uint32_t* head_of_slot; // by slot: head index, NONE for other particles

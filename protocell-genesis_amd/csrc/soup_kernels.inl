@@ -332,6 +332,34 @@ __global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
   if (t < c.n && b.head_flag[t]) b.head_rows[b.head_pos[t]] = b.head_of_slot[t];
 }
 
+// largest WCA cut of any kind pair (also the largest bond-formation and catalyst distance)
+__device__ __forceinline__ float wca_reach(const SoupConfig& c) {
+  float wmax = 0.0f;
+  for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) wmax = fmaxf(wmax, c.radius[k]);
+  return c.wca_factor * c.sigma * wmax;
+}
+// no term acts at or beyond max(WCA cuts, attraction shell, DH near/far split): partners there are skin
+__device__ __forceinline__ float force_reach(const SoupConfig& c) {
+  return fmaxf(fmaxf(wca_reach(c), c.wca_factor * c.b_tt + c.wc), c.es_split);
+}
+// Near/far split of a list row. A row keeps the entries closer than list_split at the rebuild in walk
+// order from its front (the near part) and the others in reverse walk order from its back (the far
+// part). A pair that acts now was within reach + 2 dmax at the rebuild (dmax = the largest
+// displacement since), so while 2 dmax <= skin/2 the near part holds every acting pair; while the
+// list is valid (dmax <= skin/2) it also holds, in walk order, every bond-formation candidate and
+// catalyst (within the largest WCA cut). The 0.002 sigma cover the f32 rounding of both distances and
+// of the measured drift.
+__device__ __forceinline__ float list_split(const SoupConfig& c) {
+  return fmaxf(force_reach(c) * 1.00005f + 0.5f * c.skin, wca_reach(c) + c.skin) + 0.002f;
+}
+// the force reads only the near part while the measured drift (squared, f32 bits) allows it
+__device__ __forceinline__ bool near_only(const SoupConfig& c, uint32_t drift_sq_bits) {
+  const float dm = sqrtf(__uint_as_float(drift_sq_bits));
+  return 2.0f * dm <= list_split(c) - 0.002f - force_reach(c) * 1.00005f;
+}
+// entry s of a row read as near part then far part (n = near entries)
+__device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t cap) { return s < n ? s : cap + n - 1u - s; }
+
 // ================================================================================================
 // Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92), one wavefront per particle.
 // Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Slots
@@ -339,8 +367,9 @@ __global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
 // (two when the row wraps in x), visited in canonical order. Lane r first sets up row r (cells lying
 // entirely beyond listRange + 0.01 sigma of the particle are dropped: they hold no entry, so the list
 // is unchanged) and a scan over the lanes lays the kept ranges end to end; the wavefront then sweeps
-// that one dense candidate range, VL_BATCH chunks of 64 per round of loads, and a ballot compacts
-// the hits in order. Needs (2R + 1)^2 <= 64 rows.
+// that one dense candidate range, VL_BATCH chunks of 64 per round of loads, and ballots compact the
+// hits in order: the near ones (list_split) from the front of the row, the far ones from its back.
+// Needs (2R + 1)^2 <= 64 rows.
 // ================================================================================================
 #define VL_BATCH 4u
 __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
@@ -415,7 +444,9 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
   const uint32_t cap = c.list_cap;
   uint32_t* row = b.vl + (size_t)t * cap;
   const uint64_t below = (1ull << lane) - 1ull;
-  uint32_t count = 0;
+  const float split = list_split(c);
+  const float split2 = split * split;
+  uint32_t cn = 0, cf = 0;  // near and far hits so far
   for (uint32_t base = 0; base < total; base += VL_BATCH * WAVE) {
     uint32_t kk[VL_BATCH];
     float3 sh[VL_BATCH];
@@ -441,24 +472,34 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
     }
 #pragma unroll
     for (uint32_t k = 0; k < VL_BATCH; ++k) {
-      bool hit = false;
+      bool hit = false, near = false;
       if (base + k * WAVE + lane < total && kk[k] != t) {
         // cheap reject through the known image shift, then the reference's exact test
         const float ax = (xj[k].x + sh[k].x) - xi.x, ay = (xj[k].y + sh[k].y) - xi.y, az = (xj[k].z + sh[k].z) - xi.z;
         const float a2 = ax * ax + ay * ay + az * az;
         hit = a2 <= sure;
         if (!hit && a2 <= cull) hit = dist_f(c, xi, xj[k]) < lr;
+        near = a2 < split2;
       }
-      const uint64_t mask = __ballot(hit);
+      const uint64_t mn = __ballot(hit && near), mf = __ballot(hit && !near);
       if (hit) {
-        const uint32_t p = count + (uint32_t)__popcll(mask & below);
-        if (p < cap) row[p] = kk[k];
+        // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row
+        if (near) {
+          const uint32_t p = cn + (uint32_t)__popcll(mn & below);
+          if (p < cap) row[p] = kk[k];
+        } else {
+          const uint32_t p = cf + (uint32_t)__popcll(mf & below);
+          if (p < cap) row[cap - 1u - p] = kk[k];
+        }
       }
-      count += (uint32_t)__popcll(mask);
+      cn += (uint32_t)__popcll(mn);
+      cf += (uint32_t)__popcll(mf);
     }
   }
   if (lane == 0) {
+    const uint32_t count = cn + cf;
     b.vcount[t] = count < cap ? count : cap;
+    b.vnear[t] = cn < cap ? cn : cap;
     if (count > cap) b.flags[0] = 1u;
   }
 }
@@ -559,11 +600,7 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   p.es_kappa_l2 = -c.es_kappa * 1.44269504f;  // exp(-kappa r) = exp2(es_kappa_l2 * r)
   p.split2 = c.es_split * c.es_split;
   p.es_rc = c.es_rc;
-  // no term acts at or beyond max(wca cuts, attraction shell, DH split): list entries there are skin
-  float wmax = 0.0f;
-  for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) wmax = fmaxf(wmax, c.radius[k]);
-  float wca_max = c.wca_factor * c.sigma * wmax;
-  float reach = fmaxf(fmaxf(wca_max, p.rc_attr + c.wc), c.es_split);
+  const float reach = force_reach(c);
   p.reach2 = reach * reach * 1.0001f;
   p.reach = reach;
   return p;
@@ -683,18 +720,25 @@ __device__ __forceinline__ float mi_sel(float a, float b, float L, float hL) {
 }
 
 // nonbondedSoup (pair.wgsl:168-204) for one pair already known to lie inside the interaction reach;
-// tb is the pair-table entry of the two kinds. f32 terms from one hardware reciprocal square root
-// (1 ulp), branch-free WCA and attraction shell, the DH-near term behind a branch, FP64 sum.
+// tb is the pair-table entry of the two kinds. The WCA core in FP64 from the exact f32 displacement
+// (its steep terms carry the largest f32 rounding, which strong pairs that nearly cancel would
+// expose); the attraction shell and the DH-near term in f32 from one hardware reciprocal square root
+// (1 ulp); FP64 sum.
 __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float dx, float dy, float dz, float qi, float qj,
                                           Acc& f) {
 #pragma clang fp contract(fast)
   const float r2 = dx * dx + dy * dy + dz * dz;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * ir;
-  const float ir2 = ir * ir;
-  const float s2 = tb.y * ir2;
-  const float s6 = s2 * s2 * s2;
-  float fr = (r2 < tb.x) ? P.eps24 * s6 * (2.0f * s6 - 1.0f) * ir2 : 0.0f;
+  double fw = 0.0;
+  if (r2 < tb.x) {
+    const double r2d = (double)dx * (double)dx + (double)dy * (double)dy + (double)dz * (double)dz;
+    const double ir2d = 1.0 / r2d;
+    const double s2 = (double)tb.y * ir2d;
+    const double s6 = s2 * s2 * s2;
+    fw = (double)P.eps24 * s6 * (2.0 * s6 - 1.0) * ir2d;
+  }
+  float fr = 0.0f;
   const float as = tb.z + ((P.acid > 0.0f && tb.w != 0.0f && ((qi != 0.0f) != (qj != 0.0f))) ? P.acid : 0.0f);
   const float x = fminf(fmaxf(P.attr_k2 * (r - P.rc_attr), 0.0f), 3.14159265f);
   const bool shell = as > 0.0f && r2 >= P.rc2_lo && r2 <= P.rc2_hi;
@@ -704,8 +748,7 @@ __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float d
     const float u = P.es_A * __builtin_amdgcn_exp2f(P.es_kappa_l2 * r) * ir;
     fr += qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir;
   }
-  if (r2 < 1e-12f) fr = 0.0f;  // r < 1e-6: no force (pair.wgsl:48)
-  const double g = (double)fr;
+  const double g = r2 < 1e-12f ? 0.0 : (double)fr + fw;  // r < 1e-6: no force (pair.wgsl:48)
   f.x += g * (double)dx;
   f.y += g * (double)dy;
   f.z += g * (double)dz;
@@ -758,9 +801,10 @@ __device__ __forceinline__ void bonded_lane(const SoupConfig& c, const PairConst
 #endif
 
 // List force: soup_force_list_main (verlet.wgsl:132-149) = bondedForce + the list sum of
-// nonbondedSoup, plus soup_es_force_far_main for charged heads, one wavefront per particle. Stage 1 sweeps the particle's list row 64 entries at a
-// time and keeps only pairs inside the interaction reach (about 30% of a list built with a 1.5 sigma
-// skin), compacting them with a ballot into an LDS ring; stage 2 runs the full pair terms on 64
+// nonbondedSoup, plus soup_es_force_far_main for charged heads, one wavefront per particle. Stage 1
+// sweeps the particle's list row -- only its near part while the measured drift allows it
+// (list_split) -- 64 entries at a time and keeps only pairs inside the interaction reach, compacting
+// them with a ballot into an LDS ring; stage 2 runs the full pair terms on 64
 // queued pairs at a time, so every lane of the expensive part does useful work. Lane partial sums
 // are FP64 and reduced across the wavefront. A particle farther than reach + 0.5 sigma from every
 // face has no pair that wraps inside the reach, so its stage 1 takes the plain difference (equal to
@@ -789,7 +833,17 @@ __global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int 
     const float4 xv = b.pos[t];
     const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
     const float qi = uni_f(b.charge[t]);
-    const uint32_t cnt = uni(b.vcount[t]);
+    // the near part alone while the drift allows it (list_split), else both parts
+    const uint32_t cn = uni(b.vnear[t]);
+    const bool all = !near_only(c, b.flags[2]);
+    const uint32_t cnt = all ? uni(b.vcount[t]) : cn;
+    if (all && cn < FORCE_BATCH * WAVE) {
+#pragma unroll
+      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
+        const uint32_t s = k * WAVE + lane;
+        if (s >= cn && s < cnt) u[k] = row[row_index(s, cn, cap)];
+      }
+    }
     const uint4 ov = b.bs[t];
     const uint4 own = make_uint4(uni(ov.x), uni(ov.y), uni(ov.z), 0u);
     const uint32_t owner = uni(b.clink[t]);
@@ -841,7 +895,7 @@ __global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int 
 #pragma unroll
       for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
         const uint32_t s = base + k * WAVE + lane;
-        u[k] = s < cnt ? row[s] : t;
+        u[k] = s < cnt ? row[row_index(s, cn, cap)] : t;
       }
     }
     if (qn != qh) {
@@ -970,7 +1024,8 @@ __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
   const float rcat = wca_cut(c, pair_b(c, ti, c.catalyst_kind));
   uint32_t catNear = 0, catId = SOUP_NONE, nCand = 0;
   uint32_t myJ = SOUP_NONE, myR = 0;  // candidate k lives in lane k
-  const uint32_t cnt = b.vcount[t];
+  // every candidate and catalyst lies in the near part of the row, which keeps the walk order
+  const uint32_t cnt = b.vnear[t];
   const uint32_t* row = b.vl + (size_t)t * c.list_cap;
   for (uint32_t base = 0; base < cnt && (catNear == 0u || nCand < SOUP_CAND_CAP); base += WAVE) {
     const uint32_t s = base + lane;
@@ -1549,10 +1604,10 @@ __global__ __launch_bounds__(256) void k_potential(SoupConfig c, DevBuf b, SoupP
   const float qi = uni_f(b.charge[t]);
   const double cut2 = pb.cutoff * pb.cutoff;
   double nb = 0.0, fene = 0.0, bend = 0.0, teth = 0.0;
-  const uint32_t cnt = uni(b.vcount[t]);
+  const uint32_t cnt = uni(b.vcount[t]), cn = uni(b.vnear[t]);
   const uint32_t* row = b.vl + (size_t)t * c.list_cap;
   for (uint32_t s = lane; s < cnt; s += WAVE) {
-    const uint32_t u = row[s];
+    const uint32_t u = row[row_index(s, cn, c.list_cap)];
     if (b.id[u] <= idi) continue;
     const float4 xj = b.pos[u];
     const double r2 = pot_dist2(c, xi, xj);
@@ -1659,14 +1714,43 @@ __global__ void k_bs_from_index(const uint32_t* src, const uint32_t* id, uint4* 
   uint32_t i = id[t];
   bs[t] = make_uint4(src[3u * i], src[3u * i + 1], src[3u * i + 2], 0u);
 }
-// Verlet list in the reference layout: out[i * cap + s] = particle index
-__global__ void k_vl_to_index(const uint32_t* vl, const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n,
-                              uint32_t cap) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  uint32_t i = id[t], cnt = vcount[t];
-  for (uint32_t s = 0; s < cap; ++s)
-    out[(size_t)i * cap + s] = s < cnt ? id[vl[(size_t)t * cap + s]] : SOUP_NONE;
+// position of list entry u of particle t in the walk order: the cell offset (dz, dy, dx), each
+// wrapped into -R..R, then the slot (ascending particle index inside a cell)
+__device__ __forceinline__ uint64_t walk_key(const SoupConfig& c, const uint32_t* cell_of, uint32_t ct, uint32_t u) {
+  const int R = (int)c.walk_radius, side = 2 * R + 1;
+  uint32_t a = ct, q = cell_of[u];
+  int key = 0, scale = 1;
+  for (int k = 0; k < 3; ++k) {
+    const int D = (int)c.dims[k];
+    int v = (int)(q % (uint32_t)D) - (int)(a % (uint32_t)D);
+    if (v > R) v -= D;
+    if (v < -R) v += D;
+    key += (v + R) * scale;
+    scale *= side;
+    a /= (uint32_t)D;
+    q /= (uint32_t)D;
+  }
+  return ((uint64_t)key << 32) | u;
+}
+// Verlet list in the reference layout: out[i * cap + s] = particle index, in the walk order (the near
+// part and the reversed far part of the stored row merged back)
+__global__ void k_vl_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t cap = c.list_cap, i = b.id[t], ct = b.cell_of[t];
+  const uint32_t* row = b.vl + (size_t)t * cap;
+  const uint32_t cnt = b.vcount[t], cn = b.vnear[t];
+  uint32_t pn = 0, pf = cn;  // next near entry, next far entry (as row_index positions)
+  for (uint32_t s = 0; s < cap; ++s) {
+    uint32_t v = SOUP_NONE;
+    if (s < cnt) {
+      const bool take_near =
+          pf >= cnt || (pn < cn && walk_key(c, b.cell_of, ct, row[pn]) < walk_key(c, b.cell_of, ct, row[row_index(pf, cn, cap)]));
+      v = take_near ? row[pn++] : row[row_index(pf++, cn, cap)];
+      v = b.id[v];
+    }
+    out[(size_t)i * cap + s] = v;
+  }
 }
 // DH list in the reference layout: entries are head indices, exported as particle indices
 __global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* head_id, uint32_t* out,

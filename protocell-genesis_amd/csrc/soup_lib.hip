@@ -457,6 +457,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.perm, n);
         DA(h, b.vl, (size_t)n * cfg->list_cap);
         DA(h, b.vcount, n);
+        DA(h, b.vnear, n);
         DA(h, b.flags, 8);
         DA(h, b.events, 2 * SOUP_MAX_RULES);
         DA(h, b.desorb, 2);
@@ -627,6 +628,7 @@ size_t soup_field_bytes(const SoupHandle* h, int field) {
     case SOUP_FIELD_BOND_RNG:
     case SOUP_FIELD_THERMO_RNG:
     case SOUP_FIELD_VERLET_COUNT:
+    case SOUP_FIELD_VERLET_NEAR:
     case SOUP_FIELD_CELL_OF:
     case SOUP_FIELD_CELL_IDX: return n * 4;
     case SOUP_FIELD_EVENTS: return (size_t)h->c.n_rules * 2 * 4;
@@ -662,13 +664,14 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
     case SOUP_FIELD_THERMO_RNG: to_index(h, h->b.trng); break;
     case SOUP_FIELD_CELL_OF: to_index(h, h->b.cell_of); break;
     case SOUP_FIELD_VERLET_COUNT: to_index(h, h->b.vcount); break;
+    case SOUP_FIELD_VERLET_NEAR: to_index(h, h->b.vnear); break;
     case SOUP_FIELD_BOND_SLOTS:
       hipLaunchKernelGGL(k_bs_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->b.bs, h->b.id,
                          reinterpret_cast<uint32_t*>(h->stage), n);
       break;
     case SOUP_FIELD_VERLET_LIST:
-      hipLaunchKernelGGL(k_vl_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->b.vl, h->b.vcount, h->b.id,
-                         reinterpret_cast<uint32_t*>(h->stage), n, h->c.list_cap);
+      hipLaunchKernelGGL(k_vl_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b,
+                         reinterpret_cast<uint32_t*>(h->stage));
       break;
     case SOUP_FIELD_ES_LIST:
       hipLaunchKernelGGL(k_es_to_index, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b.es, h->b.es_count,
@@ -695,7 +698,11 @@ int soup_write(SoupHandle* h, int field, const void* srcp, size_t bytes) {
   HC(h, hipMemcpyAsync(h->stage, srcp, bytes, hipMemcpyHostToDevice, h->stream));
   dim3 g(blocks(n, 256)), bl(256);
   switch (field) {
-    case SOUP_FIELD_POS: hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.pos, n); break;
+    case SOUP_FIELD_POS:
+      hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.pos, n);
+      // the list force reads the far part of a row only when the drift since the rebuild asks for it
+      hipLaunchKernelGGL(k_max_drift, g, bl, 0, h->stream, h->c, h->b);
+      break;
     case SOUP_FIELD_VEL: hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.vel, n); break;
     case SOUP_FIELD_CHARGE: hipLaunchKernelGGL(k_from_index<float>, g, bl, 0, h->stream, (const float*)h->stage, h->b.id, h->b.charge, n); break;
     case SOUP_FIELD_CENTER_LINK: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.clink, n); break;

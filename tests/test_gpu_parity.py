@@ -46,6 +46,62 @@ def test_forces_match_fp64(broth):
     assert force_rel_err(fb, f64) < FORCE_TOL
 
 
+def _list_split(cfg):
+    """Near/far split of a stored list row: max(force reach + skin/2, largest WCA cut + skin) + 0.002."""
+    wca_max = cfg.wca_factor * cfg.sigma * max(cfg.radius)
+    reach = max(wca_max, cfg.wca_factor * cfg.b_tt + cfg.wc, cfg.es_split)
+    return max(reach * 1.00005 + 0.5 * cfg.skin, wca_max + cfg.skin) + 0.002
+
+
+def test_forces_and_bonds_between_rebuilds():
+    """A stored list row keeps its near part (closer than the split at the rebuild) at the front; the
+    force reads it alone while the measured drift allows and matches the FP64 recomputation at every
+    drift up to the next rebuild, and a bond pass in mid-interval, which reads only the near part,
+    matches the oracle's walk of the whole row in lockstep from the same state."""
+    opts = dict(BROTH, seed=3, start={"C": 900, "O": 300, "H": 300, "M": 60, "W": 0})
+    R = ps.resolve(opts)
+    s = ps.createSoup(opts)
+    try:
+        every = R.cfg.rebuild_every
+        s.step(25 + (-25 - s.steps) % every)
+        assert s.steps % every == 0
+        s.step(1)  # this step rebuilt the list: the drift starts from 0
+        # the near part of each row holds the entries closer than the split at the rebuild (those
+        # within 1e-4 of it may go either way)
+        pos0 = s.particles().reshape(-1, 4)[:, :3].astype(np.float64)
+        rows, cnt, near = s.verletList(), s.verletCount(), s.readDEBUG("VERLET_NEAR")
+        split, box = _list_split(R.cfg), np.asarray(R.cfg.box, np.float64)
+        for i in range(R.N):
+            d = pos0[rows[i, : cnt[i]].astype(np.int64)] - pos0[i]
+            d = np.sqrt(((d - box * np.rint(d / box)) ** 2).sum(1))
+            assert (d < split - 1e-4).sum() <= near[i] <= (d < split + 1e-4).sum(), i
+        assert 0 < near.sum() < cnt.sum()
+        drifts = []
+        for k in range(1, every):
+            s.step(1)
+            drifts.append(float(np.sqrt(s.readDEBUG("MAX_DRIFT_SQ", np.float32)[0])))
+            o = oracle_from_system(s, R)
+            f = s.forcesNoRebuildDEBUG().reshape(-1, 4)[:, :3]
+            assert force_rel_err(f, o.force_f64(use_list=False)) < FORCE_TOL, (k, drifts[-1])
+        assert drifts[-1] > 0.2, drifts
+        s.step((-s.steps) % every)
+        o = oracle_from_system(s, R)
+        o.force[:] = s.forcesNoRebuildDEBUG()  # the first kick of the lockstep uses the current force
+        for _ in range(every // 2):
+            o.one_step()
+        s.step(every // 2)
+        assert np.array_equal(s.rngState()["bond"], o.bond_rng)
+        s.runPhasesDEBUG("BONDS")
+        o.bond_form()
+        o.bond_break()
+        assert np.array_equal(s.bondSlots(), o.bond_slots)
+        assert np.array_equal(s.centerLinks(), o.center_link)
+        assert np.array_equal(s.rngState()["bond"], o.bond_rng)
+        assert np.array_equal(s.readDEBUG("EVENTS"), o.events[: 2 * R.cfg.n_rules])
+    finally:
+        s.dispose()
+
+
 def test_integrator_bit_exact_given_force(broth):
     R, s = broth
     s.forces()
@@ -64,6 +120,7 @@ def test_integrator_bit_exact_given_force(broth):
 def test_steps_keep_rng_streams(broth):
     R, s = broth
     o = oracle_from_system(s, R)
+    o.force[:] = s.forcesNoRebuildDEBUG()  # the first kick of the lockstep uses the current force
     s.step(40)
     for _ in range(40):
         o.one_step()
