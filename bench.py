@@ -153,6 +153,7 @@ def main():
     from protocell_soup.analysis import AnalysisContext, detectStage
 
     actx = AnalysisContext()
+    reads0 = s.readDEBUG("LIST_READS").astype(np.int64)
     s.profile(True, phases=["force"])
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -172,6 +173,7 @@ def main():
     elapsed = time.perf_counter() - t_start
     prof = s.profileRead()
     steps_per_s = args.steps / elapsed
+    reads = s.readDEBUG("LIST_READS").astype(np.int64) - reads0  # near parts only, whole rows
 
     # phase breakdown from a separate, fully bracketed run (its events cost a little time per step)
     s.profile(True)
@@ -183,20 +185,29 @@ def main():
     f = prof["force"]
     force_ms = f["ms"] / max(1, f["count"])
     lbar = occ["verletMean"]
+    lnear = float(s.readDEBUG("VERLET_NEAR").mean())
+    frac_whole = float(reads[1]) / max(1, int(reads.sum()))
+    lread = frac_whole * lbar + (1.0 - frac_whole) * lnear  # Verlet entries a force launch reads
     n_heads = R.es_heads
-    # algorithmic bytes of one force launch: per particle its own position, charge, count, bond slots,
-    # centre link and force (56 B) and 4 B per Verlet entry; per charged head its DH count and 4 B per
-    # DH entry; every neighbour gather is counted once per particle (the table is read, not re-read)
-    bytes_per_launch = R.N * (56.0 + 4.0 * lbar) + n_heads * (4.0 + 4.0 * occ["esMean"])
+    # algorithmic bytes of one force launch: per particle its own position, charge, counts, bond slots,
+    # centre link and force (60 B) and 4 B per Verlet entry it reads (the near part only while the
+    # drift allows, else the whole row); per charged head its DH count and 4 B per DH entry; every
+    # neighbour gather is counted once per particle (the table is read, not re-read)
+    bytes_per_launch = R.N * (60.0 + 4.0 * lread) + n_heads * (4.0 + 4.0 * occ["esMean"])
     achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
-    bytes_step = R.N * (245.4 + 4.6 * lbar) + n_heads * (40.0 + 4.4 * occ["esMean"])
+    # per step: the force launch, the list build's writes every 10 steps and the bond pass's reads of
+    # the near parts every 20 steps, plus the per-particle state every phase streams
+    bytes_step = R.N * (249.4 + 4.0 * lread + 0.4 * lbar + 0.2 * lnear) + n_heads * (40.0 + 4.4 * occ["esMean"])
     pair_per_step = occ["verletPairs"] + occ["esPairs"]
     log("phase ms/launch: " + ", ".join(f"{k} {v['ms'] / max(1, v['count']):.3f}x{v['count']}" for k, v in phases.items() if v["count"]))
+    # measured HBM bytes of a launch that reads the near parts only and of one that reads whole rows
+    # (rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy calibration), weighted by this run's mix
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_force_list_campaign483k_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        traffic = round((1.0 - frac_whole) * tj["hbm_bytes_per_launch_near_rows"] + frac_whole * tj["hbm_bytes_per_launch_whole_rows"])
 
     cpu = None
     if not args.no_cpu:
@@ -220,7 +231,8 @@ def main():
         "config": {
             "workload": "campaign soup 76 sigma, 483,268 beads, pH 7.0, 0.01 M, long-range DH, bond MC/20, rebuild/10, "
                         "pH sweep/1000, aggregate decomposition + encapsulated-water flood/1000",
-            "N": R.N, "box": 76, "mean_verlet": round(lbar, 2), "mean_dh_list": round(occ["esMean"], 2),
+            "N": R.N, "box": 76, "mean_verlet": round(lbar, 2), "mean_verlet_near": round(lnear, 2),
+            "force_launches_whole_rows": round(frac_whole, 4), "mean_dh_list": round(occ["esMean"], 2),
             "pair_interactions_per_s": round(pair_per_step * steps_per_s, 1),
             "algorithmic_bytes_per_step": round(bytes_step),
             "step_roofline_frac": round(bytes_step * steps_per_s / (HBM_PEAK_GBS * 1e9), 4),

@@ -137,9 +137,11 @@ enum {
   SOUP_FIELD_CELL_IDX = 18,    /* u32 x n: particle indices in cell-sorted order */
   SOUP_FIELD_MAX_DRIFT_SQ = 19,/* f32 x1 */
   SOUP_FIELD_HEAD_IDX = 20,    /* u32 x n_heads */
-  SOUP_FIELD_VERLET_NEAR = 21  /* u32 per particle: entries of the stored list row closer than the near/far
+  SOUP_FIELD_VERLET_NEAR = 21, /* u32 per particle: entries of the stored list row closer than the near/far
                                   split at the rebuild (no counterpart in the reference, whose row is one
                                   walk-ordered run; SOUP_FIELD_VERLET_LIST returns that run) */
+  SOUP_FIELD_LIST_READS = 22   /* u32 x2 since creation: list-force evaluations that read the near parts
+                                  only, and those that read whole rows (no counterpart in the reference) */
 };
 
 /* Modes for soup_forces (soup/src/soup-readback.ts:32-65). */

@@ -69,7 +69,8 @@ struct DevBuf {
   uint32_t* es;
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,
-  // [4] nonfinite vel, [5] bond event count, [6..7] spare
+  // [4] nonfinite vel, [5] bond event count, [6] bond resolution did not converge, [7] spare,
+  // [8] list-force launches that read the near parts only, [9] those that read whole rows
   uint32_t* flags;
   uint32_t* events;   // 2 * rules
   uint32_t* desorb;   // 2

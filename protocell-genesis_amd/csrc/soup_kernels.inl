@@ -836,6 +836,7 @@ __global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int 
     // the near part alone while the drift allows it (list_split), else both parts
     const uint32_t cn = uni(b.vnear[t]);
     const bool all = !near_only(c, b.flags[2]);
+    if (t == 0 && lane == 0) atomicAdd(&b.flags[all ? 9 : 8], 1u);
     const uint32_t cnt = all ? uni(b.vcount[t]) : cn;
     if (all && cn < FORCE_BATCH * WAVE) {
 #pragma unroll

@@ -458,7 +458,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.vl, (size_t)n * cfg->list_cap);
         DA(h, b.vcount, n);
         DA(h, b.vnear, n);
-        DA(h, b.flags, 8);
+        DA(h, b.flags, 10);
         DA(h, b.events, 2 * SOUP_MAX_RULES);
         DA(h, b.desorb, 2);
         DA(h, b.props, (size_t)n * SOUP_CAND_CAP);
@@ -524,7 +524,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         HC(h, hipMemcpy(b.id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         HC(h, hipMemcpy(b.inv, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         HC(h, hipMemset(b.force, 0, (size_t)n * 16));
-        HC(h, hipMemset(b.flags, 0, 32));
+        HC(h, hipMemset(b.flags, 0, 40));
         uint32_t ev[2 * SOUP_MAX_RULES] = {0};
         if (init->events)
           for (uint32_t k = 0; k < 2 * cfg->n_rules && k < 2 * SOUP_MAX_RULES; ++k) ev[k] = init->events[k];
@@ -638,6 +638,7 @@ size_t soup_field_bytes(const SoupHandle* h, int field) {
     case SOUP_FIELD_ES_LIST: return (size_t)h->nh * h->c.es_cap * 4;
     case SOUP_FIELD_CELL_START: return ((size_t)h->ncells + 1) * 4;
     case SOUP_FIELD_MAX_DRIFT_SQ: return 4;
+    case SOUP_FIELD_LIST_READS: return 8;
     case SOUP_FIELD_HEAD_IDX: return (size_t)h->nh * 4;
     default: return 0;
   }
@@ -684,6 +685,7 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
     case SOUP_FIELD_EVENTS: src = h->b.events; break;
     case SOUP_FIELD_DESORB: src = h->b.desorb; break;
     case SOUP_FIELD_MAX_DRIFT_SQ: src = h->b.flags + 2; break;
+    case SOUP_FIELD_LIST_READS: src = h->b.flags + 8; break;
     default: return fail(h, SOUP_E_INVALID, fmt("soup_read: unknown field %d", field));
   }
   HC(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
