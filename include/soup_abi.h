@@ -245,6 +245,30 @@ typedef struct SoupPotentialBasis {
  * rounding). */
 int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]);
 
+/* soupPotential (soup/src/soup-potential.ts:145-301) of an arbitrary configuration of this system:
+ * positions (x, y, z, kind per particle, particle order, host memory) in the periodic box `box`,
+ * with the system's current bond slots, centre links and charges. Replaces the CPU energy of the area
+ * move's trials (soup/src/soup-area-move.ts:211-226): the pairs come from a cell grid of the
+ * configuration itself (cells of at least basis->cutoff, the reference's deduplicated 27-cell walk),
+ * the DH pairs of esTotalEnergy (electrostatics.ts:575-600) from all charged heads; FP64, fixed-order
+ * sums. out: nonbonded (incl. DH), fene, bend, tether, total. */
+int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* basis, const float* positions, const double box[3],
+                      double out[5]);
+
+/* The device half of applyBoxScaleOnce (soup/src/soup-box-scale.ts:149-201): adopts geometry->box,
+ * ->dims and ->walk_radius (the host's planSoupGrid for the new box; every other field keeps its
+ * value), writes `positions` (particle order, already mapped into the new box by
+ * soup_scale_molecules_rigid), reallocates the cell grid if it grew, and runs a rebuild and a force
+ * pass so the next step's first half-kick sees F(x) of the new geometry. */
+int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions);
+
+/* scaleMoleculesRigid (soup/src/soup-box-scale-math.ts:63-138), host code: every connected
+ * component of the bond graph (`bond_pairs`, n_pairs (i, j) pairs as SoupSystem.bonds() lists them;
+ * an unbonded particle is its own component) moves rigidly with its centre of mass scaled by
+ * new_box/old_box per axis. out may not alias positions. No device work. */
+int soup_scale_molecules_rigid(const float* positions, uint32_t n, const uint32_t* bond_pairs, uint32_t n_pairs,
+                               const double old_box[3], const double new_box[3], float* out);
+
 /* ---------------------------------------------------------------------------------------------
  * Snapshot analysis (soup/src/stages.ts:204-289's per-sample work): an analysis context holds one
  * particle snapshot in device memory -- copied from a live system or from host arrays -- and answers

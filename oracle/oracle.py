@@ -245,7 +245,7 @@ def soup_potential(basis, positions, bond_slots, center_link, charges, box):
     bs = np.ascontiguousarray(bond_slots, dtype=np.uint32).reshape(-1)
     cl = None if center_link is None else np.ascontiguousarray(center_link, dtype=np.uint32)
     ch = None if charges is None else np.ascontiguousarray(charges, dtype=np.float32)
-    bx = np.ascontiguousarray(box, dtype=np.float32)
+    bx = np.ascontiguousarray(box, dtype=np.float64)  # the reference's box is a JS number
     out = (C.c_double * 5)()
     lib().or_soup_potential(C.byref(basis), C.c_uint32(n), pos.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p),
                             None if cl is None else cl.ctypes.data_as(C.c_void_p),

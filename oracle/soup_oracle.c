@@ -655,7 +655,7 @@ static inline double pot_round(double x) { /* JS Math.round: floor(x + 0.5) sema
   return r;
 }
 static inline double pot_mi(double d, double L) { return d - pot_round(d / L) * L; } /* electrostatics.ts:408-410 */
-static double pot_dist(const float* pos, uint32_t i, uint32_t j, const float* box) {
+static double pot_dist(const float* pos, uint32_t i, uint32_t j, const double* box) {
   double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
   double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
   double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
@@ -673,7 +673,7 @@ static double pot_attr_v(const SoupPotentialBasis* b, double r) { /* soup-potent
 }
 
 void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos, const uint32_t* bond_slots,
-                       const uint32_t* center_link, const float* charges, const float* box, double out[5]) {
+                       const uint32_t* center_link, const float* charges, const double* box, double out[5]) {
   double cut2 = b->cutoff * b->cutoff, nonbonded = 0.0;
   for (uint32_t i = 0; i < n; ++i) { /* soup-potential.ts:199-240 */
     uint32_t ki = (uint32_t)pos[4u * i + 3];

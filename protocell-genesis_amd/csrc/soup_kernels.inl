@@ -1689,6 +1689,156 @@ __global__ __launch_bounds__(1024) void k_potential_sum(const double4* part, uin
   }
 }
 
+// ================================================================================================
+// soupPotential of an arbitrary configuration (soup/src/soup-potential.ts:145-301), for the area
+// move's trial boxes (soup/src/soup-area-move.ts:211-226): positions in particle order in their own
+// periodic box, with the system's bond slots, centre links and charges. The pairs come from the
+// configuration's own cell grid, as the reference builds it: cells of at least `cutoff` per axis
+// (floor(L / cutoff), at least 1), each cell walking the deduplicated set of its 27 neighbours, so a
+// pair is seen once from each side and taken from the lower particle index. The DH pairs of
+// esTotalEnergy (electrostatics.ts:575-600) are taken over all charged heads.
+// ================================================================================================
+struct PaGrid {
+  double L[3], w[3];
+  uint32_t d[3];
+};
+__device__ __forceinline__ double pa_mi(double d, double L) { return d - js_round(d / L) * L; }
+__device__ __forceinline__ double pa_dist2(const PaGrid& g, float4 a, float4 b) {
+  const double dx = pa_mi((double)a.x - (double)b.x, g.L[0]);
+  const double dy = pa_mi((double)a.y - (double)b.y, g.L[1]);
+  const double dz = pa_mi((double)a.z - (double)b.z, g.L[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ uint32_t pa_axis(double x, double L, double w, uint32_t d) {
+  const double v = x - floor(x / L) * L;
+  const double c = floor(v / w);
+  return c < (double)(d - 1u) ? (uint32_t)c : d - 1u;
+}
+// nonbonded pair energy of soup-potential.ts:230-252: WCA core plus the class-table attraction with
+// the acid-soap increment
+__device__ __forceinline__ double pot_pair(const SoupPotentialBasis& pb, uint32_t ki, uint32_t kj, float qi, float qj, double r) {
+  double e = 0.0;
+  const double bb = (pb.b_radius[ki] + pb.b_radius[kj]) / 2.0;
+  if (r < pb.wca_factor * bb) {
+    const double sr = bb / r, s2 = sr * sr, s6 = s2 * s2 * s2, q = 2.0 * s6 - 1.0;
+    e += pb.epsilon * q * q;
+  }
+  double sc = pb.attr[pb.cls[ki] & 3u][pb.cls[kj] & 3u];
+  if (pb.acid_soap > 0.0 && pb.use_charges && pb.cls[ki] == 1u && pb.cls[kj] == 1u && ((qi == 0.0f) != (qj == 0.0f)))
+    sc += pb.acid_soap;
+  if (sc > 0.0) e += sc * pot_attr(pb, r);
+  return e;
+}
+
+__global__ __launch_bounds__(256) void k_pa_cells(const float4* xs, uint32_t n, PaGrid g, uint32_t* cell_of, uint32_t* counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = xs[i];
+  const uint32_t c = pa_axis(x.x, g.L[0], g.w[0], g.d[0]) +
+                     g.d[0] * (pa_axis(x.y, g.L[1], g.w[1], g.d[1]) + g.d[1] * pa_axis(x.z, g.L[2], g.w[2], g.d[2]));
+  cell_of[i] = c;
+  atomicAdd(&counts[c], 1u);
+}
+__global__ __launch_bounds__(256) void k_pa_fill(const uint32_t* cell_of, uint32_t n, uint32_t* cursor, uint32_t* order) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) order[atomicAdd(&cursor[cell_of[i]], 1u)] = i;
+}
+// each cell's entries in ascending particle index (insertion sort; cells hold tens of particles)
+__global__ __launch_bounds__(256) void k_pa_sort(const uint32_t* start, uint32_t ncell, uint32_t* order) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t a = start[c], e = start[c + 1];
+  for (uint32_t k = a + 1; k < e; ++k) {
+    const uint32_t v = order[k];
+    uint32_t m = k;
+    while (m > a && order[m - 1] > v) {
+      order[m] = order[m - 1];
+      --m;
+    }
+    order[m] = v;
+  }
+}
+
+// one wavefront per particle i; part[i] = (nonbonded, fene, bend, tether)
+__global__ __launch_bounds__(256) void k_potential_at(SoupConfig c, DevBuf b, SoupPotentialBasis pb, const float4* xs, PaGrid g,
+                                                      const uint32_t* cell_of, const uint32_t* start, const uint32_t* order,
+                                                      double4* part, uint32_t nh) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const uint32_t i = uni(blockIdx.x * (blockDim.x / WAVE) + w);
+  if (i >= c.n) return;
+  const float4 xv = xs[i];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
+  const uint32_t ti = uni(b.inv[i]);
+  const uint32_t ki = (uint32_t)xi.w & (SOUP_MAX_KINDS - 1u);
+  const float qi = uni_f(b.charge[ti]);
+  const double cut2 = pb.cutoff * pb.cutoff;
+  // the deduplicated neighbour cells, in the reference's dz, dy, dx order of first occurrence
+  const uint32_t ci = uni(cell_of[i]);
+  const uint32_t cx = ci % g.d[0], cy = (ci / g.d[0]) % g.d[1], cz = ci / (g.d[0] * g.d[1]);
+  uint32_t nbr = SOUP_NONE;
+  if (lane < 27u) {
+    const int dz = (int)(lane / 9u) - 1, dy = (int)((lane / 3u) % 3u) - 1, dx = (int)(lane % 3u) - 1;
+    const uint32_t nz = (uint32_t)(((int)cz + dz + (int)g.d[2]) % (int)g.d[2]);
+    const uint32_t ny = (uint32_t)(((int)cy + dy + (int)g.d[1]) % (int)g.d[1]);
+    const uint32_t nx = (uint32_t)(((int)cx + dx + (int)g.d[0]) % (int)g.d[0]);
+    nbr = nx + g.d[0] * (ny + g.d[1] * nz);
+  }
+  bool first = lane < 27u;
+  for (uint32_t k = 0; k < 27u; ++k) {
+    const uint32_t v = (uint32_t)__shfl((int)nbr, (int)k, WAVE);
+    if (k < lane && v == nbr) first = false;
+  }
+  uint64_t cells = __ballot(first);
+  double nb = 0.0, fene = 0.0, bend = 0.0, teth = 0.0;
+  while (cells) {
+    const int k = __ffsll((unsigned long long)cells) - 1;
+    cells &= cells - 1ull;
+    const uint32_t cc = (uint32_t)__shfl((int)nbr, k, WAVE);
+    const uint32_t e0 = start[cc], e1 = start[cc + 1];
+    for (uint32_t s = e0 + lane; s < e1; s += WAVE) {
+      const uint32_t j = order[s];
+      if (j <= i) continue;
+      const float4 xj = xs[j];
+      const double r2 = pa_dist2(g, xi, xj);
+      if (r2 > cut2 || r2 <= 0.0) continue;
+      nb += pot_pair(pb, ki, (uint32_t)xj.w & (SOUP_MAX_KINDS - 1u), qi, b.charge[b.inv[j]], sqrt(r2));
+    }
+  }
+  if (lane == 0) {
+    const uint4 own = b.bs[ti];
+    const uint32_t sl[3] = {own.x, own.y, own.z};
+    for (int k = 0; k < 3; ++k) {
+      if (sl[k] == SOUP_NONE) continue;
+      const float4 xk = xs[sl[k]];
+      if (sl[k] > i) fene += pot_fene(pb, sqrt(pa_dist2(g, xi, xk)));
+      for (int m = k + 1; m < 3; ++m) {
+        if (sl[m] == SOUP_NONE) continue;
+        const double d = sqrt(pa_dist2(g, xk, xs[sl[m]])) - pb.bend_r0;
+        bend += 0.5 * pb.bend_k * d * d;
+      }
+    }
+    const uint32_t owner = b.clink[ti];
+    if (pb.use_center_link && owner != SOUP_NONE && owner > i) teth += pot_fene(pb, sqrt(pa_dist2(g, xi, xs[owner])));
+  }
+  if (pb.use_charges && qi != 0.0f && ki == pb.charged_kind) {
+    const double ecut2 = pb.es_cutoff * pb.es_cutoff;
+    for (uint32_t h = lane; h < nh; h += WAVE) {
+      const uint32_t j = b.head_id[h];
+      if (j <= i) continue;
+      const float qj = b.charge[b.inv[j]];
+      if (qj == 0.0f) continue;
+      const double r2 = pa_dist2(g, xi, xs[j]);
+      if (r2 >= ecut2) continue;
+      const double r = sqrt(r2);
+      if (r <= 0.0) continue;
+      const double u1 = (pb.es_A * exp(-pb.es_kappa * r)) / r;
+      nb += (double)qi * (double)qj * (u1 - pb.es_shiftU + pb.es_shiftF * (r - pb.es_cutoff));
+    }
+  }
+  nb = wave_sum_d_dpp(nb);
+  if (lane == 0) part[i] = make_double4(nb, fene, bend, teth);
+}
+
 template <typename T>
 __global__ void k_to_index(const T* src, const uint32_t* id, T* dst, uint32_t n) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;

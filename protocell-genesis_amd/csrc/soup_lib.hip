@@ -6,6 +6,7 @@
 // per 1000-step chunk: the non-finite guard, the Verlet/DH guards and the due constant-pH sweep).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -50,6 +51,11 @@ struct SoupHandle {
   uint64_t global_step = 0, next_sweep_at = UINT64_MAX;
   uint32_t prng = 0;
   uint32_t sweeps = 0;
+  size_t ncells_cap = 0;     // cell_count / cell_start capacity (a box change may need more)
+  // soup_potential_at scratch: positions in particle order, the configuration's cell grid
+  float4* pa_xs = nullptr;
+  uint32_t *pa_cell = nullptr, *pa_order = nullptr, *pa_count = nullptr, *pa_start = nullptr;
+  size_t pa_ncell_cap = 0;
   SoupSweepResult last_sweep{};
   std::string err;
   std::vector<void*> allocs;
@@ -131,6 +137,27 @@ static int dalloc(SoupHandle* h, T** p, size_t count) {
   h->allocs.push_back(q);
   *p = reinterpret_cast<T*>(q);
   return 0;
+}
+
+// frees a dalloc'd buffer (the stream must be idle)
+static void dfree(SoupHandle* h, void* p) {
+  if (!p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+  if (it != h->allocs.end()) {
+    (void)hipFree(p);
+    h->allocs.erase(it);
+  }
+}
+// a buffer of at least `count` elements, reallocated (contents dropped) when `cap` is smaller
+template <typename T>
+static int dgrow(SoupHandle* h, T** p, size_t count, size_t& cap) {
+  if (*p && count <= cap) return 0;
+  dfree(h, *p);
+  *p = nullptr;
+  cap = 0;
+  int r = dalloc(h, p, count);
+  if (!r) cap = count;
+  return r;
 }
 
 #define DA(h, p, cnt)                        \
@@ -452,6 +479,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.cell_of, n);
         DA(h, b.cell_count, h->ncells);
         DA(h, b.cell_start, (size_t)h->ncells + 1);
+        h->ncells_cap = h->ncells;
         DA(h, b.cell_rank, n);
         DA(h, b.cell_tmp, n);
         DA(h, b.perm, n);
@@ -829,6 +857,165 @@ int soup_potential(SoupHandle* h, const SoupPotentialBasis* pb, double out[5]) {
   hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, part, h->n, dout);
   HC(h, hipMemcpyAsync(out, dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return sync(h);
+}
+
+int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3], double out[5]) {
+  if (!h) return SOUP_E_INVALID;
+  if (!pb || !positions || !box || !out) return fail(h, SOUP_E_INVALID, "soup_potential_at: null argument");
+  if (!(pb->cutoff > 0.0)) return fail(h, SOUP_E_INVALID, "soup_potential_at: the pair cutoff must be > 0");
+  PaGrid g;
+  size_t ncell = 1;
+  for (int a = 0; a < 3; ++a) {
+    if (!(box[a] > 0.0) || !std::isfinite(box[a])) return fail(h, SOUP_E_INVALID, "soup_potential_at: box edges must be finite and > 0");
+    const double d = std::floor(box[a] / pb->cutoff);  // soup-potential.ts:156-158
+    g.d[a] = d < 1.0 ? 1u : (uint32_t)std::min(d, 1024.0);
+    g.L[a] = box[a];
+    g.w[a] = box[a] / (double)g.d[a];
+    ncell *= g.d[a];
+  }
+  const uint32_t n = h->n;
+  for (uint32_t i = 0; i < n; ++i) {
+    const float k = positions[4u * i + 3];
+    if (!(k >= 0.0f) || k >= (float)h->c.n_kinds || !std::isfinite(positions[4u * i]) || !std::isfinite(positions[4u * i + 1]) ||
+        !std::isfinite(positions[4u * i + 2]))
+      return fail(h, SOUP_E_INVALID, fmt("soup_potential_at: particle %u has a non-finite position or a kind outside the table", i));
+  }
+  const size_t need = (size_t)n * sizeof(double4) + 64;
+  if (need > h->stage_bytes) return fail(h, SOUP_E_INVALID, "soup_potential_at: staging buffer too small");
+  int r = sync(h);
+  if (r) return r;
+  if (!h->pa_xs) {
+    DA(h, h->pa_xs, n);
+    DA(h, h->pa_cell, n);
+    DA(h, h->pa_order, n);
+  }
+  size_t cap_c = h->pa_ncell_cap;
+  r = dgrow(h, &h->pa_count, ncell + 1, cap_c);
+  if (r) return r;
+  size_t cap_s = h->pa_ncell_cap;
+  r = dgrow(h, &h->pa_start, ncell + 1, cap_s);
+  if (r) return r;
+  h->pa_ncell_cap = std::min(cap_c, cap_s);
+  HC(h, hipMemcpyAsync(h->pa_xs, positions, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+  fill(h, h->pa_count, 0u, (uint32_t)ncell);
+  hipLaunchKernelGGL(k_pa_cells, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_xs, n, g, h->pa_cell, h->pa_count);
+  scan(h, h->pa_count, (uint32_t)ncell, h->pa_start);
+  HC(h, hipMemcpyAsync(h->pa_count, h->pa_start, ncell * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
+  hipLaunchKernelGGL(k_pa_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_cell, n, h->pa_count, h->pa_order);
+  hipLaunchKernelGGL(k_pa_sort, dim3(blocks(ncell, 256)), dim3(256), 0, h->stream, h->pa_start, (uint32_t)ncell, h->pa_order);
+  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
+  double4* part = reinterpret_cast<double4*>(h->stage);
+  double* dout = reinterpret_cast<double*>(h->stage + (size_t)n * sizeof(double4));
+  hipLaunchKernelGGL(k_potential_at, dim3(blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, h->pa_xs, g,
+                     h->pa_cell, h->pa_start, h->pa_order, part, es ? h->nh : 0u);
+  hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, part, n, dout);
+  HC(h, hipMemcpyAsync(out, dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+
+int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions) {
+  if (!h) return SOUP_E_INVALID;
+  if (!geometry || !positions) return fail(h, SOUP_E_INVALID, "soup_set_box: null geometry or positions");
+  SoupConfig c = h->c;
+  for (int a = 0; a < 3; ++a) {
+    c.box[a] = geometry->box[a];
+    c.dims[a] = geometry->dims[a];
+    if (!(c.box[a] > 0.0f) || c.dims[a] == 0)
+      return fail(h, SOUP_E_INVALID, "soup_set_box: box edges and grid dims must be > 0");
+  }
+  c.walk_radius = geometry->walk_radius;
+  int v = validate(&c);
+  if (v) return fail(h, v, g_create_error);
+  const uint32_t n = h->n;
+  for (uint32_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      // wrapped in FP64 and rounded to f32, a coordinate may land on L itself
+      const float x = positions[4u * i + a];
+      if (!(x >= 0.0f && x <= c.box[a]))
+        return fail(h, SOUP_E_INVALID, fmt("soup_set_box: particle %u coordinate %d = %g lies outside [0, %g]", i, a, x, c.box[a]));
+    }
+  int r = sync(h);
+  if (r) return r;
+  const size_t ncells = (size_t)c.dims[0] * c.dims[1] * c.dims[2];
+  if (ncells > h->ncells_cap) {
+    size_t cc = h->ncells_cap, cs = h->ncells_cap;
+    r = dgrow(h, &h->b.cell_count, ncells, cc);
+    if (r) return r;
+    r = dgrow(h, &h->b.cell_start, ncells + 1, cs);
+    if (r) return r;
+    h->ncells_cap = ncells;
+  }
+  h->c = c;
+  h->ncells = (uint32_t)ncells;
+  r = soup_write(h, SOUP_FIELD_POS, positions, (size_t)n * 16);
+  if (r) return r;
+  ph_rebuild(h);
+  ph_force(h);
+  r = sync(h);
+  if (r) return r;
+  return assert_lists(h);
+}
+
+// scaleMoleculesRigid (soup/src/soup-box-scale-math.ts:63-138): host code, FP64 in the reference's
+// order. Components of the bond graph by BFS from the lowest unvisited index, neighbours in the order
+// the pairs list them; each member's offset from the root accumulated one minimum-image hop at a
+// time; the centre of mass (mean offset, summed in BFS order) wrapped and scaled per axis, and every
+// member rebuilt around it from its unchanged offset.
+int soup_scale_molecules_rigid(const float* positions, uint32_t n, const uint32_t* bond_pairs, uint32_t n_pairs,
+                               const double old_box[3], const double new_box[3], float* out) {
+  if (!positions || !old_box || !new_box || !out || (n_pairs && !bond_pairs)) return SOUP_E_INVALID;
+  std::vector<uint32_t> deg(n + 1, 0);
+  for (uint32_t k = 0; k < n_pairs; ++k) {
+    const uint32_t a = bond_pairs[2u * k], b = bond_pairs[2u * k + 1];
+    if (a >= n || b >= n) return SOUP_E_INVALID;
+    deg[a + 1]++;
+    deg[b + 1]++;
+  }
+  for (uint32_t i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<uint32_t> adj(deg[n]), fillp(deg.begin(), deg.end() - 1);
+  for (uint32_t k = 0; k < n_pairs; ++k) {
+    const uint32_t a = bond_pairs[2u * k], b = bond_pairs[2u * k + 1];
+    adj[fillp[a]++] = b;
+    adj[fillp[b]++] = a;
+  }
+  auto mi1 = [](double d, double L) { return d - std::floor(d / L + 0.5) * L; };  // Math.round
+  auto wrap1 = [](double v, double L) { return v - std::floor(v / L) * L; };
+  const double s[3] = {new_box[0] / old_box[0], new_box[1] / old_box[1], new_box[2] / old_box[2]};
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<double> off(3 * (size_t)n, 0.0);
+  std::vector<uint32_t> members;
+  for (uint32_t root = 0; root < n; ++root) {
+    if (seen[root]) continue;
+    seen[root] = 1;
+    off[3u * root] = off[3u * root + 1] = off[3u * root + 2] = 0.0;
+    members.clear();
+    members.push_back(root);
+    for (size_t q = 0; q < members.size(); ++q) {
+      const uint32_t p = members[q];
+      for (uint32_t e = deg[p]; e < deg[p + 1]; ++e) {
+        const uint32_t c = adj[e];
+        if (seen[c]) continue;
+        seen[c] = 1;
+        for (int a = 0; a < 3; ++a)
+          off[3u * c + a] = off[3u * p + a] + mi1((double)positions[4u * c + a] - (double)positions[4u * p + a], old_box[a]);
+        members.push_back(c);
+      }
+    }
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (uint32_t m : members)
+      for (int a = 0; a < 3; ++a) sum[a] += off[3u * m + a];
+    const double cnt = (double)members.size();
+    double cen[3], com[3];
+    for (int a = 0; a < 3; ++a) {
+      cen[a] = sum[a] / cnt;
+      com[a] = wrap1((double)positions[4u * root + a] + cen[a], old_box[a]) * s[a];
+    }
+    for (uint32_t m : members) {
+      for (int a = 0; a < 3; ++a) out[4u * m + a] = (float)wrap1(com[a] + (off[3u * m + a] - cen[a]), new_box[a]);
+      out[4u * m + 3] = positions[4u * m + 3];
+    }
+  }
+  return 0;
 }
 
 int soup_relax(SoupHandle* h, uint32_t iterations, float d0) {

@@ -406,6 +406,29 @@ class SoupSystem:
     def verletCount(self) -> np.ndarray:
         return self._read("VERLET_COUNT", np.uint32)
 
+    # -- zero-tension area move (soup/src/soup-area-move.ts:167-279) -------------------------------
+    def areaMove(self, trials: int, opts: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
+        """`trials` Metropolis box proposals (mode from data/soup.json's areaMove unless opts['mode']);
+        the trial energies are soupPotential on the GPU, the accepted box is applied once at the end.
+        Returns {trials, accepted, acceptedFraction, box, lateralTrajectory, energyStart, energyEnd}."""
+        from .area_move import SoupAreaMove
+
+        if getattr(self, "_area", None) is None:
+            self._area = SoupAreaMove(self)
+        return self._area(trials, opts)
+
+    def potentialAt(self, positions: np.ndarray, box, basis: Optional[native.SoupPotentialBasis] = None) -> Dict[str, float]:
+        """soupPotential of an arbitrary configuration of this system (positions in particle order, its
+        own periodic box), with the system's bonds, centre links and charges, on the GPU."""
+        b = basis if basis is not None else self.potentialBasis()
+        out = (C.c_double * 5)()
+        bx = (C.c_double * 3)(*[float(x) for x in box])
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        if pos.size != 4 * self.N:
+            raise ValueError(f"potentialAt: positions hold {pos.size // 4} particles, the system {self.N}")
+        self._call(self._lib.soup_potential_at, C.byref(b), pos.ctypes.data_as(C.c_void_p), bx, out)
+        return {"nonbonded": out[0], "fene": out[1], "bend": out[2], "tether": out[3], "total": out[4]}
+
     # -- cold start (soup/src/soup-relax.ts:141-185) ---------------------------------------------
     def relaxColdStart(self, opts: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
         opts = opts or {}
