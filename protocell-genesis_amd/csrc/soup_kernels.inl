@@ -367,7 +367,8 @@ __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t c
 // (two when the row wraps in x), visited in canonical order. Lane r first sets up row r (cells lying
 // entirely beyond listRange + 0.01 sigma of the particle are dropped: they hold no entry, so the list
 // is unchanged) and a scan over the lanes lays the kept ranges end to end; the wavefront then sweeps
-// that one dense candidate range, VL_BATCH chunks of 64 per round of loads, and ballots compact the
+// that one dense candidate range, VL_BATCH chunks of 64 per round of loads (each lane carries its
+// row from chunk to chunk, since its index only grows), and ballots compact the
 // hits in order: the near ones (list_split) from the front of the row, the far ones from its back.
 // Needs (2R + 1)^2 <= 64 rows.
 // ================================================================================================
@@ -447,6 +448,9 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
   const float split = list_split(c);
   const float split2 = split * split;
   uint32_t cn = 0, cf = 0;  // near and far hits so far
+  // each lane's row and that row's start: a lane's index only grows (by 64 a chunk), so the row is
+  // carried from chunk to chunk and advanced past the rows that end at or before the index
+  uint32_t r = 0, rstart = 0, rend = s_end[w][0];
   for (uint32_t base = 0; base < total; base += VL_BATCH * WAVE) {
     uint32_t kk[VL_BATCH];
     float3 sh[VL_BATCH];
@@ -457,11 +461,11 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
       kk[k] = t;
       sh[k] = make_float3(0.0f, 0.0f, 0.0f);
       if (f < total) {
-        uint32_t r = 0;  // number of rows that end at or before f = the row holding f
-#pragma unroll
-        for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
-          if (r + step <= nrows && s_end[w][r + step - 1] <= f) r += step;
-        const uint32_t loc = f - (r > 0 ? s_end[w][r - 1] : 0u);
+        while (rend <= f) {  // f < total = s_end[w][nrows - 1], so r stays below nrows
+          rstart = rend;
+          rend = s_end[w][++r];
+        }
+        const uint32_t loc = f - rstart;
         const uint4 sg = s_seg[w][r];
         const float4 sv = s_sh[w][r];
         const bool in_a = loc < sg.y;
