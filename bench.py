@@ -15,6 +15,10 @@ length) + heads * (4 + 4 * mean DH length)) with its average duration measured b
 library's stream over the timed region, and the CPU baseline: the oracle's OpenMP step on the host
 cores, timed on a bounded sample of the same state. A separate fully bracketed run of
 --phase-steps steps gives the per-phase breakdown.
+
+The line's "batched" object is the replica sweep: --replicas amphiphile soups of uneven size stepped
+--replica-steps steps one replica after the other, then as many steps all together (stepReplicas),
+both as replica-steps/s.
 """
 import argparse
 import json
@@ -41,6 +45,52 @@ REFERENCE_STEPS_PER_S = 11.5  # 87.2 ms/step at 483,268 beads (reference closeou
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def replica_sweep(ps, torch, count, steps):
+    """The batched replica sweep against the per-replica loop on the same replicas: ``count``
+    amphiphile soups of uneven size (protocell_soup.replicas.sweepReplicaParams: edge 14.7-40 sigma
+    at density 0.80, pH 5-9, ionic strength 0.001-0.1 M, supply 1.5-4.8 closure thresholds, seed
+    1000 + r), each relaxed and warmed up, then ``steps`` steps of every replica one replica after
+    the other (SoupSystem.step), then ``steps`` more of all of them together (stepReplicas)."""
+    from protocell_soup.replicas import sweepReplicaOpts, sweepReplicaParams
+
+    params = sweepReplicaParams(count)
+    t0 = time.perf_counter()
+    reps = []
+    for p in params:
+        s = ps.createSoup(sweepReplicaOpts(p))
+        s.relaxColdStart()
+        reps.append(s)
+    sizes = [s.N for s in reps]
+    log(f"replicas: {count} created and relaxed, N {min(sizes)}..{max(sizes)}, total {sum(sizes)} ({time.perf_counter() - t0:.1f} s)")
+    for s in reps:
+        s.step(40)
+    ps.stepReplicas(reps, 40)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k, s in enumerate(reps):
+        s.step(steps)
+        if k % 16 == 15:
+            log(f"per-replica loop: {k + 1}/{count}")
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t
+    log(f"per-replica loop: {count * steps / t_loop:.0f} replica-steps/s ({t_loop:.2f} s)")
+    t = time.perf_counter()
+    ps.stepReplicas(reps, steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t
+    log(f"batched: {count * steps / t_batch:.0f} replica-steps/s ({t_batch:.2f} s)")
+    for s in reps:
+        s.dispose()
+    loop_rate, batch_rate = count * steps / t_loop, count * steps / t_batch
+    return {"replica_steps_per_s": round(batch_rate, 1), "loop_replica_steps_per_s": round(loop_rate, 1),
+            "speedup_vs_loop": round(batch_rate / loop_rate, 3), "replicas": count, "steps_per_replica": steps,
+            "beads_total": sum(sizes), "beads_min": min(sizes), "beads_max": max(sizes),
+            "bead_steps_per_s": round(sum(sizes) * steps / t_batch, 1),
+            "workload": "amphiphile soups at density 0.80: edge U(14.735, 40) sigma, pH U(5, 9), ionic strength "
+                        "logU(0.001, 0.1) M, supply U(1.5, 4.8) x 930/76^3 amphiphiles per sigma^3, seed 1000+r, "
+                        "charged heads with long-range DH, bond MC/20, rebuild/10, pH sweep/1000"}
 
 
 def cpu_baseline(sys_, R, sample_steps):
@@ -108,6 +158,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=20)
     ap.add_argument("--phase-steps", type=int, default=1000, help="steps of the separate per-phase profile")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--replicas", type=int, default=64, help="replicas of the batched sweep (0: skip it)")
+    ap.add_argument("--replica-steps", type=int, default=2000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale reference) and exit")
     args = ap.parse_args()
@@ -215,8 +267,14 @@ def main():
         cpu = cpu_baseline(s, R, args.cpu_sample)
         log(f"cpu baseline {cpu['value']:.3f} steps/s on {cpu['cores']} threads")
 
+    actx.close()
+    s.dispose()
+    batched = None
+    if args.replicas > 0:
+        batched = replica_sweep(ps, torch, args.replicas, args.replica_steps)
+
     out = {
-        "metric": "MD steps/s at 483k beads",
+        "metric": "MD steps/s at 483k beads; batched replica-steps/s vs per-replica loop, 1 GPU",
         "value": round(steps_per_s, 3),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -254,9 +312,8 @@ def main():
             "bytes_per_launch": round(bytes_per_launch),
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
-    actx.close()
-    s.dispose()
     print(json.dumps(out), flush=True)
 
 

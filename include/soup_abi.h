@@ -186,6 +186,13 @@ const char* soup_last_error(const SoupHandle* h);
 /* SoupSystem.step(n) (soup/src/soup-integrate.ts:136-175): n integration steps in chunks of 1000,
  * with the non-finite and Verlet guards and the due constant-pH sweep at each chunk end. */
 int soup_step(SoupHandle* h, uint32_t n);
+/* `count` systems advanced by n steps together (the replica sweep): every system's steps of a
+ * 1,000-step chunk are queued first, each system on its own stream, round robin by step period,
+ * then each system's chunk-end guards and due constant-pH sweep run. Every system ends exactly as
+ * soup_step(hs[k], n) would leave it. On failure *failed is the index of the system whose guard
+ * failed (its soup_last_error holds the message). No counterpart in the reference, which steps one
+ * system per page. */
+int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t n, uint32_t* failed);
 /* Global step counter (SoupSystem.steps). */
 int soup_global_step(const SoupHandle* h, uint64_t* out);
 
@@ -346,6 +353,12 @@ enum {
 };
 int soup_profile(SoupHandle* h, int mask);
 int soup_profile_read(SoupHandle* h, double* ms, uint64_t* count);
+
+/* Whether soup_step replays its steps from captured HIP graphs (default on; off while a profile
+ * runs): one graph per period of lcm(rebuildEvery, bondAttemptInterval) steps, so a small system
+ * is not bound by launch overhead. Results are bit-identical either way. No counterpart in the
+ * reference (WebGPU encodes its own command buffers); for tests and timing. */
+int soup_use_graphs(SoupHandle* h, int enable);
 
 /* Library build/version string. */
 const char* soup_version(void);

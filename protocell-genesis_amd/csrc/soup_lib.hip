@@ -60,6 +60,12 @@ struct SoupHandle {
   std::string err;
   std::vector<void*> allocs;
   Prof prof;
+  // captured step periods (soup_step): one per assignment of the double-buffered state arrays
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    float4* key = nullptr;  // b.pos when captured
+  } graphs[2];
+  bool use_graphs = true;
 };
 
 static hipEvent_t prof_event(SoupHandle* h) {
@@ -275,6 +281,65 @@ static void one_step(SoupHandle* h) {
   if (do_bonds) ph_bonds(h);
   ph_kick_thermo(h);
   h->global_step++;
+}
+
+// Steps are replayed from captured HIP graphs one period at a time: lcm(rebuildEvery,
+// bondAttemptInterval) steps, doubled when that holds an odd number of rebuilds, so that the
+// double-buffered state arrays end a period where they began and the graph's baked pointers stay
+// valid. A period starts where global_step is a multiple of its length.
+static uint32_t gcd_u(uint32_t a, uint32_t b) { return b ? gcd_u(b, a % b) : a; }
+static uint32_t step_period(const SoupConfig& c) {
+  const uint64_t p = (uint64_t)c.rebuild_every / gcd_u(c.rebuild_every, c.bond_interval) * c.bond_interval;
+  const uint64_t q = (p / c.rebuild_every) % 2 ? 2 * p : p;
+  return q <= 1000u ? (uint32_t)q : 0u;  // 0: no replay (a period longer than a chunk)
+}
+static void drop_graphs(SoupHandle* h) {
+  for (auto& g : h->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = SoupHandle::StepGraph{};
+  }
+}
+// one period of steps through the graph captured for the current buffer assignment (captured and
+// instantiated on first use); false if capture is unavailable, with the host state untouched
+static bool run_period(SoupHandle* h, uint32_t P) {
+  SoupHandle::StepGraph* g = nullptr;
+  for (auto& s : h->graphs)
+    if (s.exec && s.key == h->b.pos) g = &s;
+  if (g) {
+    if (hipGraphLaunch(g->exec, h->stream) != hipSuccess) return false;
+    h->global_step += P;  // the host side of those steps: an even number of buffer swaps undoes itself
+    return true;
+  }
+  g = h->graphs[0].exec ? &h->graphs[1] : &h->graphs[0];
+  if (g->exec) {
+    (void)hipGraphExecDestroy(g->exec);
+    *g = SoupHandle::StepGraph{};
+  }
+  float4* key = h->b.pos;
+  const uint64_t step0 = h->global_step;
+  const DevBuf b0 = h->b, a0 = h->alt;
+  if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  for (uint32_t k = 0; k < P; ++k) one_step(h);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+  hipGraphExec_t exec = nullptr;
+  const bool ok = ec == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  // nothing ran while capturing: restore the host state, then run the period from the graph
+  h->global_step = step0;
+  h->b = b0;
+  h->alt = a0;
+  (void)hipGetLastError();
+  if (!ok) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    h->use_graphs = false;
+    return false;
+  }
+  g->exec = exec;
+  g->key = key;
+  if (hipGraphLaunch(exec, h->stream) != hipSuccess) return false;
+  h->global_step += P;
+  return true;
 }
 
 static int sync(SoupHandle* h) {
@@ -580,6 +645,15 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
   return 0;
 }
 
+int soup_use_graphs(SoupHandle* h, int enable) {
+  if (!h) return SOUP_E_INVALID;
+  int r = sync(h);
+  if (r) return r;
+  h->use_graphs = enable != 0;
+  if (!h->use_graphs) drop_graphs(h);
+  return 0;
+}
+
 int soup_profile(SoupHandle* h, int enable) {
   if (!h) return SOUP_E_INVALID;
   int r = sync(h);
@@ -608,6 +682,7 @@ int soup_destroy(SoupHandle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->prof.pending.clear();
   for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
+  drop_graphs(h);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -620,21 +695,82 @@ int soup_global_step(const SoupHandle* h, uint64_t* out) {
   return 0;
 }
 
+// the chunk-end work of soup_step: guards, then the due constant-pH sweep
+static int chunk_end(SoupHandle* h, uint64_t chunk_start) {
+  int r = assert_finite(h, true, chunk_start);
+  if (r) return r;
+  r = assert_lists(h);
+  if (r) return r;
+  if (h->c.es_enabled && h->global_step >= h->next_sweep_at) {
+    h->next_sweep_at = h->global_step + h->c.sweep_every;
+    r = sweep(h, nullptr);
+    if (r) return r;
+  }
+  return 0;
+}
+
 int soup_step(SoupHandle* h, uint32_t nsteps) {
   if (!h) return SOUP_E_INVALID;
   uint32_t done = 0;
   while (done < nsteps) {
     uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
     uint64_t chunk_start = h->global_step;
-    for (uint32_t k = 0; k < chunk; ++k) one_step(h);
-    int r = assert_finite(h, true, chunk_start);
+    const uint32_t P = h->use_graphs && h->prof.mask == 0 ? step_period(h->c) : 0u;
+    for (uint32_t k = 0; k < chunk;) {
+      if (P && h->global_step % P == 0 && chunk - k >= P && run_period(h, P)) {
+        k += P;
+        continue;
+      }
+      one_step(h);
+      ++k;
+    }
+    int r = chunk_end(h, chunk_start);
     if (r) return r;
-    r = assert_lists(h);
-    if (r) return r;
-    if (h->c.es_enabled && h->global_step >= h->next_sweep_at) {
-      h->next_sweep_at = h->global_step + h->c.sweep_every;
-      r = sweep(h, nullptr);
-      if (r) return r;
+    done += chunk;
+  }
+  return 0;
+}
+
+int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t nsteps, uint32_t* failed) {
+  if (!hs || (count && !failed)) return SOUP_E_INVALID;
+  for (uint32_t s = 0; s < count; ++s)
+    if (!hs[s]) return SOUP_E_INVALID;
+  std::vector<uint64_t> start(count);
+  std::vector<uint32_t> left(count), period(count);
+  uint32_t done = 0;
+  while (done < nsteps) {
+    const uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
+    for (uint32_t s = 0; s < count; ++s) {
+      SoupHandle* h = hs[s];
+      start[s] = h->global_step;
+      left[s] = chunk;
+      period[s] = h->use_graphs && h->prof.mask == 0 ? step_period(h->c) : 0u;
+    }
+    // round robin: each system advances to its next period boundary or by one period per pass, so
+    // the streams of all systems have work queued together
+    for (bool busy = true; busy;) {
+      busy = false;
+      for (uint32_t s = 0; s < count; ++s) {
+        SoupHandle* h = hs[s];
+        const uint32_t P = period[s];
+        if (left[s] == 0) continue;
+        busy = true;
+        if (P && h->global_step % P == 0 && left[s] >= P && run_period(h, P)) {
+          left[s] -= P;
+          continue;
+        }
+        do {
+          one_step(h);
+          --left[s];
+        } while (left[s] && !(P && h->global_step % P == 0));
+      }
+    }
+    for (uint32_t s = 0; s < count; ++s) {
+      const int r = chunk_end(hs[s], start[s]);
+      if (r) {
+        *failed = s;
+        return r;
+      }
     }
     done += chunk;
   }
@@ -945,6 +1081,7 @@ int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positio
     if (r) return r;
     h->ncells_cap = ncells;
   }
+  drop_graphs(h);  // their kernels carry the old geometry
   h->c = c;
   h->ncells = (uint32_t)ncells;
   r = soup_write(h, SOUP_FIELD_POS, positions, (size_t)n * 16);

@@ -94,11 +94,11 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 
 # every entry point include/soup_abi.h declares
 EXPORTS = (
-    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_global_step", "soup_read", "soup_write",
+    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
     "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_set_box", "soup_scale_molecules_rigid", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
-    "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_version",
+    "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_use_graphs", "soup_version",
 )
 PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_far", "drift", "bonds", "kick_thermo",
                "proton")
@@ -120,6 +120,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_destroy": (C.c_int, [H]),
         "soup_last_error": (C.c_char_p, [H]),
         "soup_step": (C.c_int, [H, C.c_uint32]),
+        "soup_step_batch": (C.c_int, [C.POINTER(H), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
         "soup_global_step": (C.c_int, [H, C.POINTER(C.c_uint64)]),
         "soup_read": (C.c_int, [H, C.c_int, C.c_void_p, C.c_size_t]),
         "soup_write": (C.c_int, [H, C.c_int, C.c_void_p, C.c_size_t]),
@@ -152,6 +153,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_analysis_cavity": (C.c_int, [H, C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_double,
                                            C.POINTER(C.c_uint64)]),
         "soup_profile": (C.c_int, [H, C.c_int]),
+        "soup_use_graphs": (C.c_int, [H, C.c_int]),
         "soup_profile_read": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
         "soup_version": (C.c_char_p, []),
     }

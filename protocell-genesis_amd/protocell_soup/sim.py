@@ -15,7 +15,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import threading
-from typing import Any, Dict, Optional
+from typing import Any, Dict, Optional, Sequence
 
 import numpy as np
 
@@ -257,6 +257,11 @@ class SoupSystem:
     def step(self, n: int) -> None:
         """n integration steps (soup/src/soup-integrate.ts:136-175)."""
         self._call(self._lib.soup_step, C.c_uint32(int(n)))
+
+    def useGraphs(self, enable: bool = True) -> None:
+        """Replay whole step periods from captured HIP graphs (on by default). The results are
+        bit-identical either way; off, every step is launched kernel by kernel."""
+        self._call(self._lib.soup_use_graphs, C.c_int(1 if enable else 0))
 
     @property
     def steps(self) -> int:
@@ -501,6 +506,34 @@ class SoupSystem:
             self.dispose()
         except Exception:
             pass
+
+
+def stepReplicas(systems: Sequence[SoupSystem], n: int) -> None:
+    """n steps of every system, queued together on their own streams (soup_step_batch). Each
+    system ends exactly where system.step(n) would leave it; this is the batched form of a replica
+    sweep's per-replica loop. On a guard failure the error names the failing replica."""
+    systems = list(systems)
+    if not systems:
+        return
+    lib = systems[0]._lib
+    held = []
+    try:
+        for s in systems:
+            if s._h is None:
+                raise RuntimeError("stepReplicas: a system is disposed")
+            if not s._lock.acquire(blocking=False):
+                raise RuntimeError("stepReplicas: concurrent calls on one system are not allowed (or a system is listed twice)")
+            held.append(s)
+        hs = (C.c_void_p * len(systems))(*[s._h for s in systems])
+        failed = C.c_uint32(0)
+        r = lib.soup_step_batch(hs, C.c_uint32(len(systems)), C.c_uint32(int(n)), C.byref(failed))
+        if r != 0:
+            k = int(failed.value)
+            msg = lib.soup_last_error(systems[k]._h) if k < len(systems) else None
+            raise SoupError(f"replica {k}: " + (msg.decode() if msg else f"libsoup_hip error {r}"), r)
+    finally:
+        for s in held:
+            s._lock.release()
 
 
 def createSoup(opts: Dict[str, Any]) -> SoupSystem:

@@ -2,7 +2,7 @@
 import numpy as np
 
 import protocell_soup as ps
-from protocell_soup.init_state import amphiphile_soup
+from protocell_soup.replicas import amphiphileSoupOpts
 from oracle import oracle as orc
 
 # small broth: catalysts, free carbons and heads, water (tests/soup-forces.test.ts:31-45 style)
@@ -18,26 +18,7 @@ def charged(box=20.0, seed=11, pH=5.0, ionic=0.01, start=None):
 
 def amphiphile_opts(box, n_amph, tail, extra_ids, seed=1, kT=1.0, electrostatics=None):
     """createSoup opts for a soup with pre-bonded amphiphiles passed through ``resume``."""
-    from protocell_soup.rules import loadSoup
-
-    soup = loadSoup()
-    ids = [m["id"] for m in soup["monomers"]]
-    head, carbon = ids.index("O"), ids.index("C")
-    extra = {ids.index(k): v for k, v in extra_ids.items()}
-    st = amphiphile_soup([box] * 3, n_amph, tail, extra, head, carbon, len(ids), seed, kT)
-    start = {k: 0 for k in ids}
-    start["O"] += n_amph
-    start["C"] += n_amph * tail
-    for k, v in extra_ids.items():
-        start[k] += v
-    n = st["positions"].size // 4
-    resume = {"globalStep": 0, "liveBox": [box] * 3, "positions": st["positions"], "velocities": st["velocities"],
-              "bondSlots": st["bondSlots"], "centerLink": st["centerLink"], "centerHeldSteps": st["centerHeldSteps"],
-              "desorbEvents": st["desorbEvents"], "bondRng": st["bondRng"], "thermoRng": st["thermoRng"], "events": {}}
-    opts = {"box": [box] * 3, "seed": seed, "kT": kT, "clay": False, "start": start, "resume": resume}
-    if electrostatics:
-        opts["electrostatics"] = electrostatics
-    return opts
+    return amphiphileSoupOpts(box, n_amph, tail, extra_ids, seed=seed, kT=kT, electrostatics=electrostatics)
 
 
 def oracle_from_system(sys_, R):
