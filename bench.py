@@ -9,10 +9,10 @@ exactly --steps integration steps of SoupSystem.step (with the per-1000-step gua
 that fall inside it) and, every 1000 steps, the snapshot analysis of the campaign (amphiphile
 recognition, aggregate decomposition, encapsulated-water flood: detectStage).
 
-Prints ONE JSON line: steps/s, the roofline of the dominant kernel (the list force kernel, which
-also sums the far DH rows of charged heads; algorithmic bytes per launch = N * (56 + 4 * mean Verlet
-length) + heads * (4 + 4 * mean DH length)) with its average duration measured by HIP events on the
-library's stream over the timed region, and the CPU baseline: the oracle's OpenMP step on the host
+Prints ONE JSON line: steps/s, the roofline of the dominant kernel (k_force_group, the list force,
+eight lanes per particle; algorithmic bytes per launch = N * (60 + 4 * Verlet entries read) +
+heads * 28) with its average duration measured by HIP events on the library's stream over the timed
+region, and the CPU baseline: the oracle's OpenMP step on the host
 cores, timed on a bounded sample of the same state. A separate fully bracketed run of
 --phase-steps steps gives the per-phase breakdown.
 
@@ -200,8 +200,8 @@ def main():
         done += k
     log(f"warmup {args.warmup} steps done ({time.perf_counter() - t0:.1f} s)")
 
-    # timed region: HIP events bracket only the force kernel (the roofline's dominant kernel), on the
-    # library's own stream
+    # timed region: HIP events bracket only the list force kernel (the roofline's dominant kernel), on
+    # the library's own stream
     from protocell_soup.analysis import AnalysisContext, detectStage
 
     actx = AnalysisContext()
@@ -241,21 +241,23 @@ def main():
     frac_whole = float(reads[1]) / max(1, int(reads.sum()))
     lread = frac_whole * lbar + (1.0 - frac_whole) * lnear  # Verlet entries a force launch reads
     n_heads = R.es_heads
-    # algorithmic bytes of one force launch: per particle its own position, charge, counts, bond slots,
-    # centre link and force (60 B) and 4 B per Verlet entry it reads (the near part only while the
-    # drift allows, else the whole row); per charged head its DH count and 4 B per DH entry; every
-    # neighbour gather is counted once per particle (the table is read, not re-read)
-    bytes_per_launch = R.N * (60.0 + 4.0 * lread) + n_heads * (4.0 + 4.0 * occ["esMean"])
+    # algorithmic bytes of one list-force launch: per particle its own position, charge, counts, bond
+    # slots, centre link and force (60 B) and 4 B per Verlet entry it reads (the near part only while
+    # the drift allows, else the whole row); per charged head its head index and FP64 far DH sum
+    # (28 B); every neighbour gather is counted once per particle (the table is read, not re-read)
+    bytes_per_launch = R.N * (60.0 + 4.0 * lread) + n_heads * 28.0
     achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
     # per step: the force launch, the list build's writes every 10 steps and the bond pass's reads of
-    # the near parts every 20 steps, plus the per-particle state every phase streams
-    bytes_step = R.N * (249.4 + 4.0 * lread + 0.4 * lbar + 0.2 * lnear) + n_heads * (40.0 + 4.4 * occ["esMean"])
+    # the near parts every 20 steps, plus the per-particle state every phase streams; per head the
+    # refresh (40 B), the far DH kernel (48 B + 4 B per DH entry), the list force's 28 B and the DH
+    # build's writes every 10 steps
+    bytes_step = R.N * (249.4 + 4.0 * lread + 0.4 * lbar + 0.2 * lnear) + n_heads * (116.0 + 4.4 * occ["esMean"])
     pair_per_step = occ["verletPairs"] + occ["esPairs"]
     log("phase ms/launch: " + ", ".join(f"{k} {v['ms'] / max(1, v['count']):.3f}x{v['count']}" for k, v in phases.items() if v["count"]))
     # measured HBM bytes of a launch that reads the near parts only and of one that reads whole rows
     # (rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy calibration), weighted by this run's mix
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_force_list_campaign483k_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "k_force_group_campaign483k_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
@@ -301,7 +303,7 @@ def main():
             "last_stage": stage,
         },
         "roofline": {
-            "kernel": "k_force_list",
+            "kernel": "k_force_group",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,

@@ -343,8 +343,8 @@ enum {
   SOUP_PROF_SORT = 1,          /* cell sort + state permutation of a rebuild */
   SOUP_PROF_ES_BUILD = 2,      /* DH head list build */
   SOUP_PROF_VERLET_BUILD = 3,  /* Verlet list build */
-  SOUP_PROF_FORCE = 4,         /* list force kernel (near list + far DH rows of charged heads) */
-  SOUP_PROF_ES_FAR = 5,        /* far DH: fused into the force kernel, stays 0 */
+  SOUP_PROF_FORCE = 4,         /* list force kernel (list pairs + bonds; adds the heads' far DH sums) */
+  SOUP_PROF_ES_FAR = 5,        /* far DH rows of the charged heads (head refresh + far kernel) */
   SOUP_PROF_DRIFT = 6,
   SOUP_PROF_BONDS = 7,         /* bond MC, all kernels */
   SOUP_PROF_KICK_THERMO = 8,

@@ -737,7 +737,11 @@ __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float d
   double fw = 0.0;
   if (r2 < tb.x) {
     const double r2d = (double)dx * (double)dx + (double)dy * (double)dy + (double)dz * (double)dz;
-    const double ir2d = 1.0 / r2d;
+    // 1 / r2d: the hardware estimate refined by two Newton-Raphson steps (e <- e (2 - r2d e)), each
+    // squaring the relative error, to within a few ulp of the correctly rounded quotient
+    double ir2d = __builtin_amdgcn_rcp(r2d);
+    ir2d = ir2d * (2.0 - r2d * ir2d);
+    ir2d = ir2d * (2.0 - r2d * ir2d);
     const double s2 = (double)tb.y * ir2d;
     const double s6 = s2 * s2 * s2;
     fw = (double)P.eps24 * s6 * (2.0 * s6 - 1.0) * ir2d;
@@ -799,144 +803,145 @@ __device__ __forceinline__ void bonded_lane(const SoupConfig& c, const PairConst
 }
 
 
-#define FORCE_Q 128u    // per-wavefront ring of in-reach pairs (< 64 pending + one sweep of 64)
-#ifndef FORCE_BATCH
-#define FORCE_BATCH 4u  // list chunks of 64 whose loads are issued together
+#ifndef FAR_BATCH
+#define FAR_BATCH 16u  // far DH chunks of 64 whose loads are issued together
 #endif
 
-// List force: soup_force_list_main (verlet.wgsl:132-149) = bondedForce + the list sum of
-// nonbondedSoup, plus soup_es_force_far_main for charged heads, one wavefront per particle. Stage 1
-// sweeps the particle's list row -- only its near part while the measured drift allows it
-// (list_split) -- 64 entries at a time and keeps only pairs inside the interaction reach, compacting
-// them with a ballot into an LDS ring; stage 2 runs the full pair terms on 64
-// queued pairs at a time, so every lane of the expensive part does useful work. Lane partial sums
-// are FP64 and reduced across the wavefront. A particle farther than reach + 0.5 sigma from every
-// face has no pair that wraps inside the reach, so its stage 1 takes the plain difference (equal to
-// the minimum image for every pair that can act); the others take the exact image of mi_acc.
-#define FAR_BATCH 4u
-__global__ __launch_bounds__(256) void k_force_list(SoupConfig c, DevBuf b, int es) {
+// FP64 sum over each aligned group of TPP lanes (TPP = 4, 8 or 16), every lane of the wavefront
+// active: butterflies inside quads, half rows and rows through DPP; every lane of a group ends with
+// the group's sum
+template <uint32_t TPP>
+__device__ __forceinline__ double group_sum_d(double v) {
+  static_assert(TPP == 4u || TPP == 8u || TPP == 16u, "groups of 4, 8 or 16 lanes");
+  v += dpp_d<0xB1>(v);                  // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);                  // quad_perm [2,3,0,1]
+  if (TPP >= 8u) v += dpp_d<0x141>(v);   // row_half_mirror
+  if (TPP >= 16u) v += dpp_d<0x140>(v);  // row_mirror
+  return v;
+}
+
+// Far DH rows of the charged heads (soup_es_force_far_main, electrostatics-long.wgsl:63-82), one
+// wavefront per head in slot order, into far_acc in FP64; the list force adds them before it rounds
+// the head's force to f32.
+__global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t nh) {
+#pragma clang fp contract(fast)
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const uint32_t r = uni(xcd_block() * (blockDim.x / WAVE) + w);
+  if (r >= nh) return;
+  const uint32_t hh = uni(b.head_rows[r]);
+  const float4 xv = b.head_q[hh];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
+  const float qi = uni_f(xv.w);
+  Acc f{0.0, 0.0, 0.0};
+  if (qi != 0.0f) {
+    const PairConst P = make_pair_const(c);
+    const uint32_t ecnt = uni(b.es_count[hh]);
+    const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+    for (uint32_t base = 0; base < ecnt; base += FAR_BATCH * WAVE) {
+      uint32_t v[FAR_BATCH];
+#pragma unroll
+      for (uint32_t k = 0; k < FAR_BATCH; ++k) {
+        const uint32_t s = base + k * WAVE + lane;
+        v[k] = s < ecnt ? erow[s] : hh;
+      }
+      float4 xj[FAR_BATCH];
+#pragma unroll
+      for (uint32_t k = 0; k < FAR_BATCH; ++k) xj[k] = b.head_q[v[k]];
+#pragma unroll
+      for (uint32_t k = 0; k < FAR_BATCH; ++k)
+        far_pair_fast(P, xi, xj[k], qi, base + k * WAVE + lane < ecnt ? xj[k].w : 0.0f, f);
+    }
+  }
+  f.x = wave_sum_d_dpp(f.x);
+  f.y = wave_sum_d_dpp(f.y);
+  f.z = wave_sum_d_dpp(f.z);
+  if (lane == 0) {
+    b.far_acc[3u * hh] = f.x;
+    b.far_acc[3u * hh + 1u] = f.y;
+    b.far_acc[3u * hh + 2u] = f.z;
+  }
+}
+
+#ifndef FORCE_TPP
+#define FORCE_TPP 8u    // lanes per particle
+#endif
+#ifndef GROUP_BATCH
+#define GROUP_BATCH 4u  // list entries per lane whose loads are issued together
+#endif
+
+// List force, FORCE_TPP lanes per particle: soup_force_list_main (verlet.wgsl:132-149) =
+// bondedForce + the list sum of nonbondedSoup, plus the far DH sum of a charged head from far_acc.
+// Lane k of a particle's group takes entries k, k + TPP, ... of its row -- only the near part while
+// the measured drift allows it (list_split) -- and runs the pair terms on those inside the
+// interaction reach; lane sums are FP64, summed over the group, and rounded to f32 once. A particle
+// farther than reach + 0.5 sigma from every face has no pair that wraps inside the reach, so it
+// takes the plain difference (equal to the minimum image for every pair that can act); the others
+// take the exact image of mi_acc.
+__global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int es) {
 #pragma clang fp contract(fast)
   __shared__ PairTab T;
-  __shared__ float4 s_q[256 / WAVE][FORCE_Q];
-  __shared__ float s_qc[256 / WAVE][FORCE_Q];
   load_pair_tab(c, T);
-  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
-  const uint32_t t = uni(xcd_block() * (blockDim.x / WAVE) + w);
-  if (t < c.n) {
-    const PairConst P = make_pair_const(c);
-    const uint32_t cap = c.list_cap;
-    const uint32_t* row = b.vl + (size_t)t * cap;
-    // the first batch of list entries is requested with the particle's own data, before its length
-    // is known (entries past the length are in the allocation and are never used)
-    uint32_t u[FORCE_BATCH];
+  const uint32_t sub = threadIdx.x & (FORCE_TPP - 1u);
+  const uint32_t g = (xcd_block() * blockDim.x + threadIdx.x) / FORCE_TPP;
+  const bool valid = g < c.n;
+  const uint32_t t = valid ? g : c.n - 1u;
+  const PairConst P = make_pair_const(c);
+  const uint32_t cap = c.list_cap;
+  const uint32_t* row = b.vl + (size_t)t * cap;
+  const float4 xi = b.pos[t];
+  const float qi = b.charge[t];
+  const uint32_t cn = b.vnear[t];
+  const bool all = !near_only(c, b.flags[2]);
+  if (g == 0 && sub == 0) atomicAdd(&b.flags[all ? 9 : 8], 1u);
+  const uint32_t cnt = valid ? (all ? b.vcount[t] : cn) : 0u;
+  const uint32_t cmax = uni(wave_max_u(cnt));
+  const uint4 own = b.bs[t];
+  const uint32_t owner = b.clink[t];
+  Acc f{0.0, 0.0, 0.0};
+  if (valid && (own.x & own.y & own.z & owner) != SOUP_NONE && sub < 4u) bonded_lane(c, P, b, t, xi, own, owner, sub, f);
+  const float4* trow = &T.e[((uint32_t)xi.w & (SOUP_MAX_KINDS - 1u)) * SOUP_MAX_KINDS];
+  const float m = P.reach + 0.5f;
+  const bool interior = xi.x >= m && xi.x <= P.bx - m && xi.y >= m && xi.y <= P.by - m && xi.z >= m && xi.z <= P.bz - m;
+  for (uint32_t base = 0; base < cmax; base += FORCE_TPP * GROUP_BATCH) {
+    uint32_t u[GROUP_BATCH];
 #pragma unroll
-    for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
-      const uint32_t s = k * WAVE + lane;
-      u[k] = s < cap ? row[s] : t;
+    for (uint32_t k = 0; k < GROUP_BATCH; ++k) {
+      const uint32_t s = base + k * FORCE_TPP + sub;
+      u[k] = s < cnt ? row[row_index(s, cn, cap)] : t;
     }
-    const float4 xv = b.pos[t];
-    const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
-    const float qi = uni_f(b.charge[t]);
-    // the near part alone while the drift allows it (list_split), else both parts
-    const uint32_t cn = uni(b.vnear[t]);
-    const bool all = !near_only(c, b.flags[2]);
-    if (t == 0 && lane == 0) atomicAdd(&b.flags[all ? 9 : 8], 1u);
-    const uint32_t cnt = all ? uni(b.vcount[t]) : cn;
-    if (all && cn < FORCE_BATCH * WAVE) {
+    float4 xj[GROUP_BATCH];
+    float qj[GROUP_BATCH];
 #pragma unroll
-      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
-        const uint32_t s = k * WAVE + lane;
-        if (s >= cn && s < cnt) u[k] = row[row_index(s, cn, cap)];
-      }
+    for (uint32_t k = 0; k < GROUP_BATCH; ++k) {
+      xj[k] = b.pos[u[k]];
+      qj[k] = b.charge[u[k]];
     }
-    const uint4 ov = b.bs[t];
-    const uint4 own = make_uint4(uni(ov.x), uni(ov.y), uni(ov.z), 0u);
-    const uint32_t owner = uni(b.clink[t]);
-    const float4* trow = &T.e[((uint32_t)xi.w & (SOUP_MAX_KINDS - 1u)) * SOUP_MAX_KINDS];
-    Acc f{0.0, 0.0, 0.0};
-    if ((own.x & own.y & own.z & owner) != SOUP_NONE && lane < 4u) bonded_lane(c, P, b, t, xi, own, owner, lane, f);
-    float4* q = s_q[w];
-    float* qc = s_qc[w];
-    const float m = P.reach + 0.5f;
-    const bool interior = xi.x >= m && xi.x <= P.bx - m && xi.y >= m && xi.y <= P.by - m && xi.z >= m && xi.z <= P.bz - m;
-    const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t qn = 0, qh = 0;
-    for (uint32_t base = 0;;) {
-      float4 xj[FORCE_BATCH];
-      float qj[FORCE_BATCH];
 #pragma unroll
-      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
-        const uint32_t uk = base + k * WAVE + lane < cnt ? u[k] : t;
-        xj[k] = b.pos[uk];
-        qj[k] = b.charge[uk];
+    for (uint32_t k = 0; k < GROUP_BATCH; ++k) {
+      float dx, dy, dz;
+      if (interior) {
+        dx = xi.x - xj[k].x; dy = xi.y - xj[k].y; dz = xi.z - xj[k].z;
+      } else {
+        dx = mi_sel(xi.x, xj[k].x, P.bx, P.hx); dy = mi_sel(xi.y, xj[k].y, P.by, P.hy); dz = mi_sel(xi.z, xj[k].z, P.bz, P.hz);
       }
-#pragma unroll
-      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
-        float dx, dy, dz;
-        if (interior) {
-          dx = xi.x - xj[k].x; dy = xi.y - xj[k].y; dz = xi.z - xj[k].z;
-        } else {
-          dx = mi_sel(xi.x, xj[k].x, P.bx, P.hx); dy = mi_sel(xi.y, xj[k].y, P.by, P.hy); dz = mi_sel(xi.z, xj[k].z, P.bz, P.hz);
-        }
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool in = base + k * WAVE + lane < cnt && r2 >= 1e-12f && r2 < P.reach2;
-        const uint64_t mask = __ballot(in);
-        if (in) {
-          const uint32_t p = (qn + (uint32_t)__popcll(mask & below)) & (FORCE_Q - 1u);
-          q[p] = make_float4(dx, dy, dz, __uint_as_float((uint32_t)xj[k].w & (SOUP_MAX_KINDS - 1u)));
-          qc[p] = qj[k];
-        }
-        qn += (uint32_t)__popcll(mask);
-        if (qn - qh >= WAVE) {
-          wave_lds_sync();
-          const uint32_t p = (qh + lane) & (FORCE_Q - 1u);
-          const float4 e = q[p];
-          pair_body(P, trow[__float_as_uint(e.w)], e.x, e.y, e.z, qi, qc[p], f);
-          qh += WAVE;
-        }
-      }
-      base += FORCE_BATCH * WAVE;
-      if (base >= cnt) break;
-#pragma unroll
-      for (uint32_t k = 0; k < FORCE_BATCH; ++k) {
-        const uint32_t s = base + k * WAVE + lane;
-        u[k] = s < cnt ? row[row_index(s, cn, cap)] : t;
-      }
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (base + k * FORCE_TPP + sub < cnt && r2 >= 1e-12f && r2 < P.reach2)
+        pair_body(P, trow[(uint32_t)xj[k].w & (SOUP_MAX_KINDS - 1u)], dx, dy, dz, qi, qj[k], f);
     }
-    if (qn != qh) {
-      wave_lds_sync();
-      if (lane < qn - qh) {
-        const uint32_t p = (qh + lane) & (FORCE_Q - 1u);
-        const float4 e = q[p];
-        pair_body(P, trow[__float_as_uint(e.w)], e.x, e.y, e.z, qi, qc[p], f);
-      }
-    }
-    // a charged head also sums its far DH row here, in the same FP64 accumulator, so its force is
-    // rounded to f32 once
-    const uint32_t hh = es ? uni(b.head_of_slot[t]) : SOUP_NONE;
-    if (hh != SOUP_NONE && qi != 0.0f) {
-      const uint32_t ecnt = uni(b.es_count[hh]);
-      const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
-      for (uint32_t base = 0; base < ecnt; base += FAR_BATCH * WAVE) {
-        uint32_t v[FAR_BATCH];
-#pragma unroll
-        for (uint32_t k = 0; k < FAR_BATCH; ++k) {
-          const uint32_t s = base + k * WAVE + lane;
-          v[k] = s < ecnt ? erow[s] : hh;
-        }
-        float4 xj[FAR_BATCH];
-#pragma unroll
-        for (uint32_t k = 0; k < FAR_BATCH; ++k) xj[k] = b.head_q[v[k]];
-#pragma unroll
-        for (uint32_t k = 0; k < FAR_BATCH; ++k)
-          far_pair_fast(P, xi, xj[k], qi, base + k * WAVE + lane < ecnt ? xj[k].w : 0.0f, f);
-      }
-    }
-    f.x = wave_sum_d_dpp(f.x);
-    f.y = wave_sum_d_dpp(f.y);
-    f.z = wave_sum_d_dpp(f.z);
-    if (lane == 0) b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
   }
+  // a charged head's far DH sum joins the same FP64 accumulator, so its force is rounded once
+  if (es && valid && sub == 0u && qi != 0.0f) {
+    const uint32_t hh = b.head_of_slot[t];
+    if (hh != SOUP_NONE) {
+      f.x += b.far_acc[3u * hh];
+      f.y += b.far_acc[3u * hh + 1u];
+      f.z += b.far_acc[3u * hh + 2u];
+    }
+  }
+  f.x = group_sum_d<FORCE_TPP>(f.x);
+  f.y = group_sum_d<FORCE_TPP>(f.y);
+  f.z = group_sum_d<FORCE_TPP>(f.z);
+  if (valid && sub == 0u) b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
 }
 
 // O(N^2) reference force: soup_force_brute_main (step.wgsl:277-292), test path only

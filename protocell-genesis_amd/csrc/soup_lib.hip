@@ -243,11 +243,17 @@ static void ph_rebuild(SoupHandle* h) {
   fill(h, h->b.flags + 2, 0u, 1);
 }
 
-// list force + far DH (one kernel: heads add their far row in the same pass)
+// far DH rows of the charged heads (FP64 per head), then the list force, which adds them to the
+// heads' sums before rounding
 static void ph_force(SoupHandle* h) {
+  if (es_on(h)) {
+    hipEvent_t e0 = prof_begin(h, SOUP_PROF_ES_FAR);
+    hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+    hipLaunchKernelGGL(k_far_dh, dim3(xcd_blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    prof_end(h, SOUP_PROF_ES_FAR, e0);
+  }
   hipEvent_t e = prof_begin(h, SOUP_PROF_FORCE);
-  if (es_on(h)) hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
-  hipLaunchKernelGGL(k_force_list, dim3(xcd_blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b,
+  hipLaunchKernelGGL(k_force_group, dim3(xcd_blocks((uint64_t)h->n * FORCE_TPP, 256)), dim3(256), 0, h->stream, h->c, h->b,
                      es_on(h) ? 1 : 0);
   prof_end(h, SOUP_PROF_FORCE, e);
 }
@@ -573,6 +579,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.head_pos, (size_t)n + 1);
         DA(h, b.head_rows, nh);
         DA(h, b.head_q, nh);
+        DA(h, b.far_acc, (size_t)nh * 3);
         if (h->nh > 0) {
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
@@ -599,6 +606,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_pos = b.head_pos;
         a.head_rows = b.head_rows;
         a.head_q = b.head_q;
+        a.far_acc = b.far_acc;
         // initial state: slot = particle index
         std::vector<uint4> bs(n);
         for (uint32_t i = 0; i < n; ++i)

@@ -17,6 +17,9 @@ DENSE = dict(BROTH, seed=5, start={"C": 600, "O": 100, "H": 500, "M": 20, "W": 2
 def _charged(box, seed, pH, ionic):
     o = charged(box=box, seed=seed, pH=pH, ionic=ionic)
     o["electrostatics"]["sweepEverySteps"] = 100
+    # DH rows sized for heads gathered into a quarter of the box: these soups aggregate, and the
+    # capacity from a uniform head density can overflow on some trajectories
+    o["electrostatics"]["densityVolumeSigma3"] = box ** 3 / 4.0
     return o
 
 
