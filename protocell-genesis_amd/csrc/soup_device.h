@@ -67,6 +67,7 @@ struct DevBuf {
   uint32_t* head_rows;    // heads in slot (spatial) order
   float4* head_q;         // by head: (x, y, z, charge), refreshed before every force evaluation
   double* far_acc;        // by head: FP64 far DH force (x, y, z), summed before the list force
+  float4* gq;             // by slot: (x, y, z, kind | 8 if charged), the list force's neighbour gather
   uint32_t* es;
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,

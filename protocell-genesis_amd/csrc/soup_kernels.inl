@@ -142,6 +142,16 @@ __device__ __forceinline__ uint32_t drift_sq_bits(const SoupConfig& c, float4 xi
   return __float_as_uint(ex * ex + ey * ey + ez * ez);
 }
 
+// the list force's neighbour record: position and, in w, the kind with bit 3 set for a charged
+// particle (every charge is 0 or chargeDeprotonated), so one 16-byte gather carries both
+__device__ __forceinline__ float4 gq_pack(float4 x, float q) {
+  return make_float4(x.x, x.y, x.z, (float)(((uint32_t)x.w & (SOUP_MAX_KINDS - 1u)) | (q != 0.0f ? SOUP_MAX_KINDS : 0u)));
+}
+__global__ __launch_bounds__(256) void k_gq_refresh(SoupConfig c, DevBuf b) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < c.n) b.gq[t] = gq_pack(b.pos[t], b.charge[t]);
+}
+
 // ================================================================================================
 // Integrator: kick_drift_wrap_main (soup/wgsl/step.wgsl:96-108),
 //             kick_thermostat_main (step.wgsl:110-142)
@@ -171,6 +181,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
       x = make_float4(px, py, pz, x.w);
       b.pos[t] = x;
     }
+    b.gq[t] = gq_pack(x, b.charge[t]);
     bits = drift_sq_bits(c, x, b.pos_at_rebuild[t]);
   }
   bits = wave_max_u(bits);
@@ -255,6 +266,7 @@ __global__ __launch_bounds__(256) void k_relax_step(SoupConfig c, DevBuf b, floa
   py = py - floorf(py / c.box[1]) * c.box[1];
   pz = pz - floorf(pz / c.box[2]) * c.box[2];
   b.pos[t] = make_float4(px, py, pz, x.w);
+  b.gq[t] = gq_pack(b.pos[t], b.charge[t]);
 }
 
 // ================================================================================================
@@ -294,7 +306,9 @@ __global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf 
   o.pos[ns] = x;
   o.pos_at_rebuild[ns] = x;  // soup_snapshot_positions_main
   o.vel[ns] = b.vel[t];
-  o.charge[ns] = b.charge[t];
+  const float q = b.charge[t];
+  o.charge[ns] = q;
+  o.gq[ns] = gq_pack(x, q);
   o.bs[ns] = b.bs[t];
   o.clink[ns] = b.clink[t];
   o.cheld[ns] = b.cheld[t];
@@ -586,7 +600,7 @@ __device__ __forceinline__ void load_pair_tab(const SoupConfig& c, PairTab& T) {
 struct PairConst {
   float bx, by, bz, hx, hy, hz;
   float eps24, rc_attr, rc2_lo, rc2_hi, attr_k2, attr_a;
-  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2, reach, es_kappa_l2;
+  float acid, es_A, es_kappa, es_Frc, split2, es_rc, reach2, reach, es_kappa_l2, qdep;
 };
 
 __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
@@ -607,6 +621,7 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   const float reach = force_reach(c);
   p.reach2 = reach * reach * 1.0001f;
   p.reach = reach;
+  p.qdep = (float)c.ph_q;
   return p;
 }
 
@@ -910,12 +925,8 @@ __global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int
       u[k] = s < cnt ? row[row_index(s, cn, cap)] : t;
     }
     float4 xj[GROUP_BATCH];
-    float qj[GROUP_BATCH];
 #pragma unroll
-    for (uint32_t k = 0; k < GROUP_BATCH; ++k) {
-      xj[k] = b.pos[u[k]];
-      qj[k] = b.charge[u[k]];
-    }
+    for (uint32_t k = 0; k < GROUP_BATCH; ++k) xj[k] = b.gq[u[k]];
 #pragma unroll
     for (uint32_t k = 0; k < GROUP_BATCH; ++k) {
       float dx, dy, dz;
@@ -925,8 +936,10 @@ __global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int
         dx = mi_sel(xi.x, xj[k].x, P.bx, P.hx); dy = mi_sel(xi.y, xj[k].y, P.by, P.hy); dz = mi_sel(xi.z, xj[k].z, P.bz, P.hz);
       }
       const float r2 = dx * dx + dy * dy + dz * dz;
-      if (base + k * FORCE_TPP + sub < cnt && r2 >= 1e-12f && r2 < P.reach2)
-        pair_body(P, trow[(uint32_t)xj[k].w & (SOUP_MAX_KINDS - 1u)], dx, dy, dz, qi, qj[k], f);
+      if (base + k * FORCE_TPP + sub < cnt && r2 >= 1e-12f && r2 < P.reach2) {
+        const uint32_t kw = (uint32_t)xj[k].w;
+        pair_body(P, trow[kw & (SOUP_MAX_KINDS - 1u)], dx, dy, dz, qi, (kw & SOUP_MAX_KINDS) ? P.qdep : 0.0f, f);
+      }
     }
   }
   // a charged head's far DH sum joins the same FP64 accumulator, so its force is rounded once

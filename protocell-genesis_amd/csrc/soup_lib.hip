@@ -509,6 +509,10 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
     uint32_t l = init->center_link[i];
     if (l != SOUP_NONE && (l >= cfg->n || l == i))
       return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: centerLink of particle %u names %u (N=%u)", i, l, cfg->n));
+    const float q = init->charges[i];
+    if (q != 0.0f && q != (float)cfg->ph_q)
+      return fail(nullptr, SOUP_E_INVALID,
+                  fmt("soup_create: charge of particle %u is %g; charges are 0 or chargeDeprotonated=%g", i, (double)q, cfg->ph_q));
   }
   SoupHandle* h = new SoupHandle();
   h->c = *cfg;
@@ -580,6 +584,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.head_rows, nh);
         DA(h, b.head_q, nh);
         DA(h, b.far_acc, (size_t)nh * 3);
+        DA(h, b.gq, n);
         if (h->nh > 0) {
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
@@ -607,6 +612,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_rows = b.head_rows;
         a.head_q = b.head_q;
         a.far_acc = b.far_acc;
+        a.gq = b.gq;
         // initial state: slot = particle index
         std::vector<uint4> bs(n);
         for (uint32_t i = 0; i < n; ++i)
@@ -876,9 +882,20 @@ int soup_write(SoupHandle* h, int field, const void* srcp, size_t bytes) {
       hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.pos, n);
       // the list force reads the far part of a row only when the drift since the rebuild asks for it
       hipLaunchKernelGGL(k_max_drift, g, bl, 0, h->stream, h->c, h->b);
+      hipLaunchKernelGGL(k_gq_refresh, g, bl, 0, h->stream, h->c, h->b);
       break;
     case SOUP_FIELD_VEL: hipLaunchKernelGGL(k_from_index<float4>, g, bl, 0, h->stream, (const float4*)h->stage, h->b.id, h->b.vel, n); break;
-    case SOUP_FIELD_CHARGE: hipLaunchKernelGGL(k_from_index<float>, g, bl, 0, h->stream, (const float*)h->stage, h->b.id, h->b.charge, n); break;
+    case SOUP_FIELD_CHARGE: {
+      // the list force's gather array carries a charge as a flag: every charge is 0 or chargeDeprotonated
+      const float* q = static_cast<const float*>(srcp);
+      for (uint32_t i = 0; i < n; ++i)
+        if (q[i] != 0.0f && q[i] != (float)h->c.ph_q)
+          return fail(h, SOUP_E_INVALID, fmt("soup_write: charge of particle %u is %g; charges are 0 or chargeDeprotonated=%g", i,
+                                             (double)q[i], h->c.ph_q));
+      hipLaunchKernelGGL(k_from_index<float>, g, bl, 0, h->stream, (const float*)h->stage, h->b.id, h->b.charge, n);
+      hipLaunchKernelGGL(k_gq_refresh, g, bl, 0, h->stream, h->c, h->b);
+      break;
+    }
     case SOUP_FIELD_CENTER_LINK: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.clink, n); break;
     case SOUP_FIELD_CENTER_HELD: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.cheld, n); break;
     case SOUP_FIELD_BOND_RNG: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.brng, n); break;
