@@ -1514,7 +1514,41 @@ __global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, ui
 }
 
 #define CHAIN_THREADS 256
+#define CHAIN_REG 8u  // row entries per thread held in registers: rows of up to 2,048 entries are prefetched
+
+// one head's pair-table row, CHAIN_REG entries per thread (entry tid + k * CHAIN_THREADS)
+struct ChainRow {
+  double2 ew[CHAIN_REG];
+  uint32_t g[CHAIN_REG];
+  uint32_t cnt;
+};
+// the head chosen by the next draw from state st (st is a copy)
+__device__ __forceinline__ uint32_t chain_site(uint32_t st, uint32_t nh) {
+  const double hf = floor(pcg_next_u(st) * (double)nh);
+  return hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
+}
+// requests row h's first `lim` entries (the table's rows are es_cap long; entries past the count are
+// loaded but never used)
+__device__ __forceinline__ void chain_load(const DevBuf& b, const double2* tab, const uint32_t* tab_g, uint32_t cap, uint32_t h,
+                                           uint32_t lim, uint32_t tid, ChainRow& r) {
+  const double2* trow = tab + (size_t)h * cap;
+  const uint32_t* grow = tab_g + (size_t)h * cap;
+#pragma unroll
+  for (uint32_t k = 0; k < CHAIN_REG; ++k) {
+    const uint32_t s = tid + k * CHAIN_THREADS;
+    if (s < lim) {
+      r.ew[k] = trow[s];
+      r.g[k] = grow[s];
+    }
+  }
+  r.cnt = b.es_count[h];
+}
+
 // stats: [0] attempts, [1] accepted, [2] deprotonated, [3] dEsCount; dstats: [0] dEsSum
+// The serial chain of protonationSweep (electrostatics.ts:435-496) in one workgroup. Every attempt
+// draws its site, and draws the acceptance number only when dG > 0, so the next attempt's site is
+// one of two heads known before this attempt's sum: both rows are requested while this attempt
+// runs, and the one the decision selects is in registers when the next attempt starts.
 __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, DevBuf b, uint32_t nh, const double2* tab,
                                                                 const uint32_t* tab_g, uint32_t* rng_io, uint32_t* stats,
                                                                 double* dstats) {
@@ -1526,19 +1560,41 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
   uint32_t state = *rng_io;
   const double kT = c.ph_kT, q = c.ph_q;
   const double dGintr = kT * 2.302585092994046 * (c.ph_pKa - c.ph_pH);
+  const uint32_t cap = c.es_cap, lim = cap < CHAIN_THREADS * CHAIN_REG ? cap : CHAIN_THREADS * CHAIN_REG;
   uint32_t accepted = 0, dEsCount = 0;
   double dEsSum = 0.0;
+  ChainRow cur, ra, rb;
+  uint32_t h = 0;
+  if (nh > 0) {
+    h = chain_site(state, nh);
+    chain_load(b, tab, tab_g, cap, h, lim, tid, cur);
+  }
   for (uint32_t a = 0; a < nh; ++a) {
-    double u = pcg_next_u(state);
-    double hf = floor(u * (double)nh);
-    uint32_t h = hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
-    const uint32_t cnt = b.es_count[h];
-    const uint32_t* row = tab_g + (size_t)h * c.es_cap;
-    const double2* trow = tab + (size_t)h * c.es_cap;
+    (void)pcg_next_u(state);  // this attempt's site draw (h)
+    uint32_t ha = 0, hb = 0;
+    if (a + 1 < nh) {
+      uint32_t s2 = state;
+      (void)pcg_next_u(s2);
+      ha = chain_site(state, nh);  // next site if this attempt draws no acceptance number
+      hb = chain_site(s2, nh);     // next site if it does
+      chain_load(b, tab, tab_g, cap, ha, lim, tid, ra);
+      chain_load(b, tab, tab_g, cap, hb, lim, tid, rb);
+    }
+    const uint32_t cnt = cur.cnt;
     double se = 0.0, sa = 0.0;
-    for (uint32_t s = tid; s < cnt; s += CHAIN_THREADS) {
-      double2 ew = trow[s];
-      float qj = qv[row[s]];
+#pragma unroll
+    for (uint32_t k = 0; k < CHAIN_REG; ++k) {
+      const uint32_t s = tid + k * CHAIN_THREADS;
+      if (s < cnt && s < lim) {
+        const float qj = qv[cur.g[k]];
+        if (qj != 0.0f) se += (q * (double)qj) * cur.ew[k].x;
+        sa += qj == 0.0f ? cur.ew[k].y : -cur.ew[k].y;
+      }
+    }
+    // rows longer than the register window: the rest straight from the table
+    for (uint32_t s = lim + tid; s < cnt; s += CHAIN_THREADS) {
+      const double2 ew = tab[(size_t)h * cap + s];
+      const float qj = qv[tab_g[(size_t)h * cap + s]];
       if (qj != 0.0f) se += (q * (double)qj) * ew.x;
       sa += qj == 0.0f ? ew.y : -ew.y;
     }
@@ -1557,8 +1613,9 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
     double uEs = te + (c.as_scale > 0.0 ? ta : 0.0);
     bool isCharged = qv[h] != 0.0f;
     double dG = isCharged ? -(dGintr + uEs) : dGintr + uEs;
-    bool acc = dG <= 0.0;
-    if (!acc) acc = pcg_next_u(state) < exp(-dG / kT);
+    const bool drew = !(dG <= 0.0);
+    bool acc = !drew;
+    if (drew) acc = pcg_next_u(state) < exp(-dG / kT);
     if (acc) {
       qv[h] = isCharged ? 0.0f : (float)q;  // every thread writes the same value to its own view
       accepted++;
@@ -1566,6 +1623,10 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
         dEsSum += uEs / kT;
         dEsCount++;
       }
+    }
+    if (a + 1 < nh) {
+      h = drew ? hb : ha;
+      cur = drew ? rb : ra;
     }
   }
   __syncthreads();
