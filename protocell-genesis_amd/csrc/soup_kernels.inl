@@ -1598,8 +1598,8 @@ __global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, De
       if (qj != 0.0f) se += (q * (double)qj) * ew.x;
       sa += qj == 0.0f ? ew.y : -ew.y;
     }
-    se = wave_sum_d(se);
-    sa = wave_sum_d(sa);
+    se = wave_sum_d_dpp(se);
+    sa = wave_sum_d_dpp(sa);
     if (lane == 0) {
       s_part[a & 1u][0][w] = se;
       s_part[a & 1u][1][w] = sa;
