@@ -262,12 +262,26 @@ int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]
 int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* basis, const float* positions, const double box[3],
                       double out[5]);
 
-/* The device half of applyBoxScaleOnce (soup/src/soup-box-scale.ts:149-201): adopts geometry->box,
- * ->dims and ->walk_radius (the host's planSoupGrid for the new box; every other field keeps its
- * value), writes `positions` (particle order, already mapped into the new box by
- * soup_scale_molecules_rigid), reallocates the cell grid if it grew, and runs a rebuild and a force
- * pass so the next step's first half-kick sees F(x) of the new geometry. */
-int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions);
+/* The device half of a box change: adopts geometry->box, ->dims and ->walk_radius (the host's
+ * planSoupGrid for the new box; every other field keeps its value), writes `positions` (particle
+ * order, already mapped into the new box) unless it is NULL, reallocates the cell grid if it grew,
+ * and rebuilds the lists.
+ * - applyBoxScaleOnce (soup/src/soup-box-scale.ts:149-201): positions from
+ *   soup_scale_molecules_rigid and flags SOUP_SET_BOX_FORCE, so the next step's first half-kick sees
+ *   F(x) of the new geometry;
+ * - growBoxTo (soup-box-scale.ts:318-373): positions NULL (nothing moves) and flags 0, a rebuild
+ *   without a force pass as the reference's. */
+enum { SOUP_SET_BOX_FORCE = 1 };
+int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions, uint32_t flags);
+
+/* proximityPairs (soup/src/soup-box-scale-math.ts:162-215), host code: the (i, j), j > i, pairs of
+ * eligible particles (eligible[i] != 0) within `cutoff` under the minimum image of `box`, found
+ * through a cell list of floor(box / cutoff) cells per axis, in the reference's order (i ascending,
+ * neighbour cells dx, dy, dz = -1..1, each cell's members ascending; an axis with fewer than 3 cells
+ * visits its cells more than once, and those pairs repeat as they do in the reference). Writes at
+ * most max_pairs pairs to pairs (2 entries each; NULL to count only) and the total to *count. */
+int soup_proximity_pairs(const float* positions, uint32_t n, const double box[3], double cutoff, const uint8_t* eligible,
+                         uint32_t* pairs, uint64_t max_pairs, uint64_t* count);
 
 /* scaleMoleculesRigid (soup/src/soup-box-scale-math.ts:63-138), host code: every connected
  * component of the bond graph (`bond_pairs`, n_pairs (i, j) pairs as SoupSystem.bonds() lists them;

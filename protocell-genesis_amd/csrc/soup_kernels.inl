@@ -306,6 +306,7 @@ __global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf 
   o.pos[ns] = x;
   o.pos_at_rebuild[ns] = x;  // soup_snapshot_positions_main
   o.vel[ns] = b.vel[t];
+  o.force[ns] = b.force[t];  // a rebuild without a force pass (growBoxTo) keeps F(x) per particle
   const float q = b.charge[t];
   o.charge[ns] = q;
   o.gq[ns] = gq_pack(x, q);

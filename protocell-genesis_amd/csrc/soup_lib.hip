@@ -218,6 +218,7 @@ static void ph_rebuild(SoupHandle* h) {
   hipLaunchKernelGGL(k_permute, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->alt, h->b.cell_rank);
   std::swap(h->b.pos, h->alt.pos);
   std::swap(h->b.vel, h->alt.vel);
+  std::swap(h->b.force, h->alt.force);
   std::swap(h->b.charge, h->alt.charge);
   std::swap(h->b.bs, h->alt.bs);
   std::swap(h->b.clink, h->alt.clink);
@@ -540,7 +541,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DevBuf& a = h->alt;
         DA(h, b.pos, n); DA(h, a.pos, n);
         DA(h, b.vel, n); DA(h, a.vel, n);
-        DA(h, b.force, n);
+        DA(h, b.force, n); DA(h, a.force, n);
         DA(h, b.pos_at_rebuild, n);
         DA(h, b.charge, n); DA(h, a.charge, n);
         DA(h, b.bs, n); DA(h, a.bs, n);
@@ -602,7 +603,6 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         h->stage_bytes = std::max<size_t>(h->stage_bytes, (size_t)n * sizeof(double4) + 64);  // soup_potential
         DA(h, h->stage, h->stage_bytes);
         // alternates share the arrays that are rewritten in place
-        a.force = b.force;
         a.pos_at_rebuild = b.pos_at_rebuild;
         a.cell_of = b.cell_of;
         a.inv = b.inv;
@@ -631,6 +631,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         HC(h, hipMemcpy(b.id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         HC(h, hipMemcpy(b.inv, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
         HC(h, hipMemset(b.force, 0, (size_t)n * 16));
+        HC(h, hipMemset(a.force, 0, (size_t)n * 16));
         HC(h, hipMemset(b.flags, 0, 40));
         uint32_t ev[2 * SOUP_MAX_RULES] = {0};
         if (init->events)
@@ -1074,9 +1075,9 @@ int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* 
   return sync(h);
 }
 
-int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions) {
+int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions, uint32_t flags) {
   if (!h) return SOUP_E_INVALID;
-  if (!geometry || !positions) return fail(h, SOUP_E_INVALID, "soup_set_box: null geometry or positions");
+  if (!geometry) return fail(h, SOUP_E_INVALID, "soup_set_box: null geometry");
   SoupConfig c = h->c;
   for (int a = 0; a < 3; ++a) {
     c.box[a] = geometry->box[a];
@@ -1088,13 +1089,20 @@ int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positio
   int v = validate(&c);
   if (v) return fail(h, v, g_create_error);
   const uint32_t n = h->n;
-  for (uint32_t i = 0; i < n; ++i)
-    for (int a = 0; a < 3; ++a) {
-      // wrapped in FP64 and rounded to f32, a coordinate may land on L itself
-      const float x = positions[4u * i + a];
-      if (!(x >= 0.0f && x <= c.box[a]))
-        return fail(h, SOUP_E_INVALID, fmt("soup_set_box: particle %u coordinate %d = %g lies outside [0, %g]", i, a, x, c.box[a]));
-    }
+  if (positions) {
+    for (uint32_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        // wrapped in FP64 and rounded to f32, a coordinate may land on L itself
+        const float x = positions[4u * i + a];
+        if (!(x >= 0.0f && x <= c.box[a]))
+          return fail(h, SOUP_E_INVALID, fmt("soup_set_box: particle %u coordinate %d = %g lies outside [0, %g]", i, a, x, c.box[a]));
+      }
+  } else {
+    // the particles stay where they are, which must lie inside the new box
+    for (int a = 0; a < 3; ++a)
+      if (c.box[a] < h->c.box[a])
+        return fail(h, SOUP_E_INVALID, "soup_set_box: keeping the positions needs a box at least as large on every axis");
+  }
   int r = sync(h);
   if (r) return r;
   const size_t ncells = (size_t)c.dims[0] * c.dims[1] * c.dims[2];
@@ -1109,13 +1117,25 @@ int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positio
   drop_graphs(h);  // their kernels carry the old geometry
   h->c = c;
   h->ncells = (uint32_t)ncells;
-  r = soup_write(h, SOUP_FIELD_POS, positions, (size_t)n * 16);
-  if (r) return r;
+  if (positions) {
+    r = soup_write(h, SOUP_FIELD_POS, positions, (size_t)n * 16);
+    if (r) return r;
+  }
   ph_rebuild(h);
-  ph_force(h);
+  if (flags & SOUP_SET_BOX_FORCE) ph_force(h);
   r = sync(h);
   if (r) return r;
+  r = assert_finite(h, false, 0);
+  if (r) return r;
   return assert_lists(h);
+}
+
+// Math.round of the reference: the nearest integer, halves toward +infinity (floor(x + 0.5) would
+// round 0.49999999999999994 up)
+static double host_js_round(double x) {
+  double r = std::floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  return r;
 }
 
 // scaleMoleculesRigid (soup/src/soup-box-scale-math.ts:63-138): host code, FP64 in the reference's
@@ -1140,7 +1160,7 @@ int soup_scale_molecules_rigid(const float* positions, uint32_t n, const uint32_
     adj[fillp[a]++] = b;
     adj[fillp[b]++] = a;
   }
-  auto mi1 = [](double d, double L) { return d - std::floor(d / L + 0.5) * L; };  // Math.round
+  auto mi1 = [](double d, double L) { return d - host_js_round(d / L) * L; };
   auto wrap1 = [](double v, double L) { return v - std::floor(v / L) * L; };
   const double s[3] = {new_box[0] / old_box[0], new_box[1] / old_box[1], new_box[2] / old_box[2]};
   std::vector<uint8_t> seen(n, 0);
@@ -1177,6 +1197,66 @@ int soup_scale_molecules_rigid(const float* positions, uint32_t n, const uint32_
       out[4u * m + 3] = positions[4u * m + 3];
     }
   }
+  return 0;
+}
+
+// proximityPairs (soup/src/soup-box-scale-math.ts:162-215): a cell list whose cells hold their members
+// in ascending index; the FP64 arithmetic of the reference (wrap1, floor, mi1)
+int soup_proximity_pairs(const float* positions, uint32_t n, const double box[3], double cutoff, const uint8_t* eligible,
+                         uint32_t* pairs, uint64_t max_pairs, uint64_t* count) {
+  if (!positions || !box || !eligible || !count || !(cutoff > 0.0)) return SOUP_E_INVALID;
+  int nc[3];
+  double w[3];
+  for (int a = 0; a < 3; ++a) {
+    if (!(box[a] > 0.0)) return SOUP_E_INVALID;
+    nc[a] = std::max(1, (int)std::floor(box[a] / cutoff));
+    w[a] = box[a] / (double)nc[a];
+  }
+  const size_t ncell = (size_t)nc[0] * nc[1] * nc[2];
+  std::vector<std::vector<uint32_t>> bucket(ncell);
+  std::vector<int> cx(n), cy(n), cz(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!eligible[i]) continue;
+    int cc[3];
+    for (int a = 0; a < 3; ++a) {
+      const double v = (double)positions[4u * i + a];
+      const double x = v - std::floor(v / box[a]) * box[a];
+      cc[a] = std::min(nc[a] - 1, (int)std::floor(x / w[a]));
+    }
+    cx[i] = cc[0];
+    cy[i] = cc[1];
+    cz[i] = cc[2];
+    bucket[(size_t)cc[0] + (size_t)nc[0] * ((size_t)cc[1] + (size_t)nc[1] * (size_t)cc[2])].push_back(i);
+  }
+  auto mi1 = [](double d, double L) { return d - host_js_round(d / L) * L; };
+  const double cut2 = cutoff * cutoff;
+  uint64_t k = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!eligible[i]) continue;
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int ncx = ((cx[i] + dx) % nc[0] + nc[0]) % nc[0];
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int ncy = ((cy[i] + dy) % nc[1] + nc[1]) % nc[1];
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int ncz = ((cz[i] + dz) % nc[2] + nc[2]) % nc[2];
+          for (uint32_t j : bucket[(size_t)ncx + (size_t)nc[0] * ((size_t)ncy + (size_t)nc[1] * (size_t)ncz)]) {
+            if (j <= i) continue;
+            const double ddx = mi1((double)positions[4u * i] - (double)positions[4u * j], box[0]);
+            const double ddy = mi1((double)positions[4u * i + 1] - (double)positions[4u * j + 1], box[1]);
+            const double ddz = mi1((double)positions[4u * i + 2] - (double)positions[4u * j + 2], box[2]);
+            if (ddx * ddx + ddy * ddy + ddz * ddz <= cut2) {
+              if (pairs && k < max_pairs) {
+                pairs[2 * k] = i;
+                pairs[2 * k + 1] = j;
+              }
+              ++k;
+            }
+          }
+        }
+      }
+    }
+  }
+  *count = k;
   return 0;
 }
 

@@ -1,5 +1,4 @@
-"""The zero-tension Metropolis area move (soup/src/soup-area-move.ts) and the box change it applies
-(applyBoxScaleOnce, soup/src/soup-box-scale.ts:81-201).
+"""The zero-tension Metropolis area move (soup/src/soup-area-move.ts).
 
 The chain runs on the host as the reference's does, one proposal at a time:
 - each trial box is mapped molecule by molecule (scaleMoleculesRigid, native host code);
@@ -7,9 +6,9 @@ The chain runs on the host as the reference's does, one proposal at a time:
   (soup_potential_at);
 - the Metropolis test uses the reference's mulberry32 stream and its Jacobian term.
 
-The accepted chain is applied to the device once, at the box it ended on: the rigid map, the
-reference's bonded-distance self-check, then soup_set_box, which adopts the new grid and runs a
-rebuild and a force pass.
+The accepted chain is applied to the device once, at the box it ended on, through
+box_scale.applyBoxScaleOnce: the rigid map, the reference's bonded-distance self-check, then
+soup_set_box, which adopts the new grid and runs a rebuild and a force pass.
 """
 from __future__ import annotations
 
@@ -21,6 +20,7 @@ import numpy as np
 
 from . import native
 from .init_state import Mulberry32
+from .box_scale import applyBoxScaleOnce  # noqa: F401  (the box change an accepted chain applies)
 from .plan import planSoupGrid
 
 Box = Tuple[float, float, float]
@@ -85,53 +85,6 @@ def scaleMoleculesRigid(positions: np.ndarray, bonds: np.ndarray, oldBox: Sequen
     if r != 0:
         raise ValueError("scaleMoleculesRigid: a bond names a particle outside the system")
     return out
-
-
-def _mi3(p: np.ndarray, box: Sequence[float], i: np.ndarray, j: np.ndarray) -> np.ndarray:
-    d = p[i, :3].astype(np.float64) - p[j, :3].astype(np.float64)
-    b = np.asarray(box, np.float64)
-    d -= np.floor(d / b + 0.5) * b
-    return np.sqrt((d * d).sum(axis=1))
-
-
-def applyBoxScaleOnce(sys_, targetBox: Sequence[float]) -> None:
-    """soup-box-scale.ts:81-201 without the cohesion edges: the rigid map of the current
-    configuration into `targetBox`, the bonded-distance self-check, the new grid from planSoupGrid,
-    then soup_set_box (positions, grid, rebuild and force on the device)."""
-    R = sys_._R
-    frozen = int((sys_.frozen() != 0).sum())
-    if frozen > 0:
-        raise ValueError(
-            f"changing the box is impossible on a system with a mineral sheet: {frozen} immobile beads "
-            "(areaMove/scaleBoxTo/dryWetCycle require CreateSoupOpts.clay=false, see data/soup.json clay.basis §7)")
-    target = tuple(float(x) for x in targetBox)
-    old = tuple(float(x) for x in R.live_box)
-    before = sys_.particles().reshape(-1, 4)
-    pairs = sys_.bonds()
-    after = scaleMoleculesRigid(before, pairs, old, target)
-    if pairs.size:
-        i, j = pairs[0::2].astype(np.int64), pairs[1::2].astype(np.int64)
-        d0 = _mi3(before, old, i, j)
-        d1 = _mi3(after.reshape(-1, 4), target, i, j)
-        bad = np.nonzero(np.abs(d1 - d0) > 1e-3)[0]
-        if bad.size:
-            k = int(bad[0])
-            raise RuntimeError(
-                f"applyBoxScale: an intramolecular distance changed ({d0[k]:.6f} -> {d1[k]:.6f}) for bond {i[k]}-{j[k]} "
-                f"at box [{','.join(str(x) for x in old)}] -> [{','.join(str(x) for x in target)}] -- scaleMoleculesRigid is broken")
-    plan = planSoupGrid(target, R.start)
-    if not plan.valid:
-        raise ValueError(plan.reason)
-    geo = native.SoupConfig()
-    C.memmove(C.byref(geo), C.byref(R.cfg), C.sizeof(native.SoupConfig))
-    for a in range(3):
-        geo.box[a] = target[a]
-        geo.dims[a] = plan.dims[a]
-    geo.walk_radius = plan.effectiveWalkRadius
-    sys_._call(sys_._lib.soup_set_box, C.byref(geo), after.ctypes.data_as(C.c_void_p))
-    R.cfg = geo
-    R.live_box = target
-    R.plan = plan
 
 
 class SoupAreaMove:

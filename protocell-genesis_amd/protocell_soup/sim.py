@@ -23,6 +23,7 @@ from . import native
 from .electrostatics import EsBasis, initialCharges, makeEsBasis
 from .init_state import NONE_U32, lattice_state
 from .native import FIELD, SoupError
+from .box_scale import applyBoxScale, cyclePhaseAt, deriveCycleConfig, growBoxTo, stepCycled
 from .plan import deriveGridGeometry, deriveListCapacity, densestDensityOf, planSoupGrid, verletListBytes
 from .rules import (
     acceptanceProbability,
@@ -63,7 +64,7 @@ class Resolved:
 
 
 def resolve(opts: Dict[str, Any]) -> Resolved:
-    """soup/src/sim.ts:96-322 restricted to unconfined, clay-free, non-cycling systems."""
+    """soup/src/sim.ts:96-322 restricted to unconfined, clay-free systems without solvent evaporation."""
     soup = loadSoup()
     if len(soup["monomers"]) > 8:
         raise ValueError(f"data/soup.json: {len(soup['monomers'])} monomer kinds, but the shader holds at most 8")
@@ -84,8 +85,9 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
         start[clay["mineralId"]] = 0
     if opts.get("confine"):
         raise ValueError("createSoup: parcel confinement is not part of this build")
-    if opts.get("dryWetCycle", soup["dryWetCycle"]["enabled"]):
-        raise ValueError("createSoup: dry-wet cycling is not part of this build")
+    cycling = bool(opts.get("dryWetCycle", soup["dryWetCycle"]["enabled"]))
+    if cycling and opts.get("evaporateSolvent", soup["dryWetCycle"].get("evaporateSolvent", False)):
+        raise ValueError("createSoup: solvent evaporation during the dry phase is not part of this build")
     counts_by_kind = [int(start.get(m["id"], 0)) for m in soup["monomers"]]
     capacity_n = sum(counts_by_kind)
     resume = opts.get("resume")
@@ -104,9 +106,9 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     plan = planSoupGrid(live_box, start)
     if not plan.valid:
         raise ValueError(plan.reason)
-    dry_box = box
+    cycle_cfg, dry_box = deriveCycleConfig(soup, opts, box, capacity_n, start)
     min_box = min(*live_box, *dry_box)
-    densest = densestDensityOf(soup, capacity_n, box, N, live_box, False)
+    densest = densestDensityOf(soup, capacity_n, box, N, live_box, cycle_cfg is not None)
     cap = deriveListCapacity(soup, geo.listRange, capacity_n, densest)
     vo = opts.get("verletOverride") or {}
     if vo.get("enabled") is False:
@@ -154,6 +156,9 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     R.init, R.charges0, R.prng, R.events0, R.initial_step = init, charges0, prng_state, events0, initial_step
     R.next_sweep_at = initial_step + sweep_every if es.enabled else 2 ** 64 - 1
     R.verlet = dict(soup["verletList"], listCapacity=list_cap)
+    R.cycle_cfg, R.dry_box = cycle_cfg, dry_box
+    ph = cyclePhaseAt(initial_step, cycle_cfg) if cycle_cfg else {"phase": "none", "cycleIndex": 0}
+    R.cycle_phase, R.cycle_index = ph["phase"], ph["cycleIndex"]
     R.cfg = build_config(R, opts)
     return R
 
@@ -275,12 +280,27 @@ class SoupSystem:
 
     @property
     def cyclePhase(self) -> str:
-        """'none': dry-wet cycling is not part of this path (soup/src/soup-types.ts cyclePhase)."""
-        return "none"
+        """'wet' / 'dry' on a dry-wet cycling system, 'none' otherwise (soup-types.ts:387-404)."""
+        return self._R.cycle_phase
 
     @property
     def cycleIndex(self) -> int:
-        return 0
+        """1-based cycle of the current phase; 0 when cycling is off or over."""
+        return self._R.cycle_index
+
+    def stepCycled(self, n: int) -> None:
+        """n steps through the dry-wet schedule (soup-box-scale.ts:478-532); a plain step(n) when the
+        system does not cycle."""
+        stepCycled(self, n)
+
+    def scaleBoxTo(self, targetBox, rampSteps: int, rampRelaxSteps: int, cohesion: Optional[Dict[str, Any]] = None) -> None:
+        """A ramped rigid-molecule box change to targetBox (soup-box-scale.ts:266-275); `cohesion` =
+        {cutoff, memberIndices?} makes proximity-connected clusters scale as one rigid body."""
+        applyBoxScale(self, targetBox, int(rampSteps), int(rampRelaxSteps), cohesion)
+
+    def growBoxTo(self, targetBox, rampSteps: int, rampRelaxSteps: int) -> None:
+        """Grows the box to targetBox without moving any particle (soup-box-scale.ts:318-373)."""
+        growBoxTo(self, targetBox, int(rampSteps), int(rampRelaxSteps))
 
     # -- readbacks (soup/src/soup-readback.ts) ----------------------------------------------------
     def particles(self) -> np.ndarray:
