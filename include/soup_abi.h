@@ -252,6 +252,23 @@ typedef struct SoupPotentialBasis {
  * rounding). */
 int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]);
 
+/* The adjoint of soupPotential (soup/src/soup-potential.ts:145-301) at the handle's configuration,
+ * in FP64, over the same pairs as soup_potential (current lists; the drift guard must hold):
+ * - grad[3 N] = dU/dx per particle (particle order), which is -F of the step's force law;
+ * - with a direction w[3 N] (particle order): hw[3 N] = H w, the Hessian of U times w, which is
+ *   -(w . dF/dx), and dFw_dtheta = d(w . F)/dtheta;
+ * - dU_dtheta = dU/dtheta.
+ * theta, SOUP_ADJ_PARAMS entries: the attraction depth of the unordered class pairs (0,0), (0,1),
+ * (0,2), (0,3), (1,1), (1,2), (1,3), (2,2), (2,3), (3,3) (basis->attr, both halves of the table
+ * together), the acid-soap depth, epsilon, the head charge (chargeDeprotonated: every charge is 0 or
+ * it) and the DH kappa at fixed coefficient and cutoff (the shifts follow kappa). w, hw and
+ * dFw_dtheta may be NULL together, and dU_dtheta may be NULL. The gradient of the reference's
+ * energy and forces with respect to positions and the parameter table, which the reference does
+ * not compute (its soupPotential is an energy only); for sensitivities and force-matching. */
+#define SOUP_ADJ_PARAMS 14
+int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* basis, const double* w, double* grad, double* hw,
+                        double* dU_dtheta, double* dFw_dtheta);
+
 /* soupPotential (soup/src/soup-potential.ts:145-301) of an arbitrary configuration of this system:
  * positions (x, y, z, kind per particle, particle order, host memory) in the periodic box `box`,
  * with the system's current bond slots, centre links and charges. Replaces the CPU energy of the area

@@ -1774,6 +1774,242 @@ __global__ __launch_bounds__(1024) void k_potential_sum(const double4* part, uin
 }
 
 // ================================================================================================
+// Adjoint of soupPotential (soup/src/soup-potential.ts:145-301) at the handle's configuration, FP64:
+// for each particle i the gradient g_i = dU/dx_i (= -F_i of the step's force law), and for a caller's
+// direction w the Hessian-vector product (H w)_i, which is -(w . dF/dx)_i, plus the partial
+// derivatives of U and of w . F with respect to the parameter table: the attraction depths of the
+// 10 unordered class pairs, the acid-soap depth, epsilon, the head charge and the DH screening
+// kappa (at fixed coefficient and cutoff). Pair terms phi(r) contribute phi' e to g_i, with e the
+// unit vector from j to i, and phi'' e (e . dw) + (phi' / r)(dw - e (e . dw)) to (H w)_i, with
+// dw = w_i - w_j; parameters take each unordered pair once (the lower particle index), as the
+// potential does. The pairs are those of soupPotential: list pairs within basis->cutoff, bonds,
+// 1-3 bends through each centre, centre-link tethers, and DH pairs of charged heads.
+// ================================================================================================
+#define ADJ_P 14u  // 10 class pairs, acid-soap, epsilon, head charge, kappa
+struct AdjAcc {
+  double g[3], hw[3], dU[ADJ_P], dF[ADJ_P];
+};
+__device__ __forceinline__ uint32_t adj_attr_index(uint32_t a, uint32_t b) {
+  const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo * 4u - (lo * (lo - 1u)) / 2u + (hi - lo);  // (0,0) 0 ... (0,3) 3, (1,1) 4 ... (3,3) 9
+}
+// accumulates the position parts of one pair term with derivatives d1 = phi'(r), d2 = phi''(r)
+__device__ __forceinline__ void adj_pos(AdjAcc& A, double d1, double d2, const double e[3], double r, const double dw[3],
+                                        double ew) {
+  const double t = d1 / r;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    A.g[a] += d1 * e[a];
+    A.hw[a] += d2 * ew * e[a] + t * (dw[a] - e[a] * ew);
+  }
+}
+__device__ __forceinline__ double adj_fene_d1(const SoupPotentialBasis& pb, double r) {
+  const double x2 = (r / pb.fene_r_inf) * (r / pb.fene_r_inf);
+  return pb.fene_k * r / (1.0 - x2);
+}
+__device__ __forceinline__ double adj_fene_d2(const SoupPotentialBasis& pb, double r) {
+  const double x2 = (r / pb.fene_r_inf) * (r / pb.fene_r_inf);
+  return pb.fene_k * (1.0 + x2) / ((1.0 - x2) * (1.0 - x2));
+}
+// displacement x_i - x_j under the FP64 minimum image, its length and unit vector
+__device__ __forceinline__ double adj_geom(const SoupConfig& c, float4 xi, float4 xj, double e[3]) {
+  e[0] = pot_mi((double)xi.x - (double)xj.x, c.box[0]);
+  e[1] = pot_mi((double)xi.y - (double)xj.y, c.box[1]);
+  e[2] = pot_mi((double)xi.z - (double)xj.z, c.box[2]);
+  return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+}
+__device__ __forceinline__ void adj_dw(const double* w, uint32_t idi, uint32_t idj, double dw[3]) {
+  if (w) {
+    dw[0] = w[3u * idi] - w[3u * idj];
+    dw[1] = w[3u * idi + 1u] - w[3u * idj + 1u];
+    dw[2] = w[3u * idi + 2u] - w[3u * idj + 2u];
+  } else {
+    dw[0] = dw[1] = dw[2] = 0.0;
+  }
+}
+// a bonded pair term (FENE: fene = true; 1-3 bend otherwise) on particle i
+__device__ __forceinline__ void adj_bonded(const SoupConfig& c, const SoupPotentialBasis& pb, float4 xi, float4 xj,
+                                           uint32_t idi, uint32_t idj, const double* w, bool fene, AdjAcc& A) {
+  double e[3];
+  const double r = sqrt(adj_geom(c, xi, xj, e));
+  if (!(r > 0.0)) return;
+  e[0] /= r; e[1] /= r; e[2] /= r;
+  double dw[3];
+  adj_dw(w, idi, idj, dw);
+  const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
+  if (fene) adj_pos(A, adj_fene_d1(pb, r), adj_fene_d2(pb, r), e, r, dw, ew);
+  else adj_pos(A, pb.bend_k * (r - pb.bend_r0), pb.bend_k, e, r, dw, ew);
+}
+
+__global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, SoupPotentialBasis pb, int es, const double* w,
+                                                        double* g_out, double* hw_out, double* part) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), wv = uni(threadIdx.x / WAVE);
+  const uint32_t t = uni(blockIdx.x * (blockDim.x / WAVE) + wv);
+  if (t >= c.n) return;
+  const float4 xv = b.pos[t];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), uni_f(xv.w));
+  const uint32_t idi = uni(b.id[t]);
+  const uint32_t ki = (uint32_t)xi.w & (SOUP_MAX_KINDS - 1u);
+  const float qi = uni_f(b.charge[t]);
+  const double cut2 = pb.cutoff * pb.cutoff;
+  const double eps = pb.epsilon, z1 = 3.141592653589793 / (2.0 * pb.wc);
+  AdjAcc A;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) A.g[a] = A.hw[a] = 0.0;
+#pragma unroll
+  for (uint32_t p = 0; p < ADJ_P; ++p) A.dU[p] = A.dF[p] = 0.0;
+  const uint32_t cnt = uni(b.vcount[t]), cn = uni(b.vnear[t]);
+  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    const uint32_t u = row[row_index(s, cn, c.list_cap)];
+    const float4 xj = b.pos[u];
+    double e[3];
+    const double r2 = adj_geom(c, xi, xj, e);
+    if (r2 > cut2 || r2 <= 0.0) continue;
+    const double r = sqrt(r2);
+    e[0] /= r; e[1] /= r; e[2] /= r;
+    const uint32_t idj = b.id[u];
+    double dw[3];
+    adj_dw(w, idi, idj, dw);
+    const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
+    const bool once = idj > idi;  // parameters: each unordered pair once
+    const uint32_t kj = (uint32_t)xj.w & (SOUP_MAX_KINDS - 1u);
+    const double bb = (pb.b_radius[ki] + pb.b_radius[kj]) / 2.0;
+    if (r < pb.wca_factor * bb) {
+      const double sr = bb / r, s2 = sr * sr, u6 = s2 * s2 * s2;
+      const double phi = eps * (2.0 * u6 - 1.0) * (2.0 * u6 - 1.0);
+      const double d1 = -24.0 * eps * u6 * (2.0 * u6 - 1.0) / r;
+      const double d2 = 24.0 * eps * u6 * (26.0 * u6 - 7.0) / r2;
+      adj_pos(A, d1, d2, e, r, dw, ew);
+      if (once) {
+        A.dU[11] += phi / eps;
+        A.dF[11] -= ew * d1 / eps;
+      }
+    }
+    const uint32_t ci = pb.cls[ki] & 3u, cj = pb.cls[kj] & 3u;
+    const bool acid = pb.acid_soap > 0.0 && pb.use_charges && pb.cls[ki] == 1u && pb.cls[kj] == 1u &&
+                      ((qi == 0.0f) != (b.charge[u] == 0.0f));
+    const double sc = pb.attr[ci][cj] + (acid ? pb.acid_soap : 0.0);
+    if (sc > 0.0 && r <= pb.rc_attr + pb.wc) {
+      double a0, a1, a2;  // A(r) = pot_attr and its derivatives (epsilon included)
+      if (r < pb.rc_attr) {
+        a0 = -eps; a1 = 0.0; a2 = 0.0;
+      } else {
+        const double z = z1 * (r - pb.rc_attr), cz = cos(z);
+        a0 = -eps * cz * cz;
+        a1 = eps * z1 * sin(2.0 * z);
+        a2 = 2.0 * eps * z1 * z1 * cos(2.0 * z);
+      }
+      adj_pos(A, sc * a1, sc * a2, e, r, dw, ew);
+      if (once) {
+        const uint32_t pi = adj_attr_index(ci, cj);
+        A.dU[pi] += a0;
+        A.dF[pi] -= ew * a1;
+        if (acid) {
+          A.dU[10] += a0;
+          A.dF[10] -= ew * a1;
+        }
+        A.dU[11] += sc * a0 / eps;
+        A.dF[11] -= ew * sc * a1 / eps;
+      }
+    }
+  }
+  if (lane == 0) {
+    const uint4 own = b.bs[t];
+    const uint32_t sl[3] = {own.x, own.y, own.z};
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t p = sl[k];
+      if (p == SOUP_NONE) continue;
+      const uint32_t up = b.inv[p];
+      adj_bonded(c, pb, xi, b.pos[up], idi, p, w, true, A);
+      // 1-3 bends through partner p: this particle and p's other partners
+      const uint4 ms = b.bs[up];
+      const uint32_t ml[3] = {ms.x, ms.y, ms.z};
+      for (int q = 0; q < 3; ++q) {
+        const uint32_t o = ml[q];
+        if (o == SOUP_NONE || o == idi) continue;
+        adj_bonded(c, pb, xi, b.pos[b.inv[o]], idi, o, w, false, A);
+      }
+    }
+    const uint32_t owner = b.clink[t];
+    if (pb.use_center_link && owner != SOUP_NONE) adj_bonded(c, pb, xi, b.pos[b.inv[owner]], idi, owner, w, true, A);
+  }
+  const uint32_t hh = es ? uni(b.head_of_slot[t]) : SOUP_NONE;
+  if (hh != SOUP_NONE && qi != 0.0f) {
+    const uint32_t ecnt = uni(b.es_count[hh]);
+    const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+    const double ecut2 = pb.es_cutoff * pb.es_cutoff, rc = pb.es_cutoff, kap = pb.es_kappa;
+    const double erc = pb.es_A * exp(-kap * rc);
+    for (uint32_t s = lane; s < ecnt; s += WAVE) {
+      const uint32_t gh = erow[s];
+      const float4 xq = b.head_q[gh];
+      if (xq.w == 0.0f) continue;
+      double e[3];
+      const double r2 = adj_geom(c, xi, xq, e);
+      if (r2 >= ecut2) continue;
+      const double r = sqrt(r2);
+      if (!(r > 0.0)) continue;
+      e[0] /= r; e[1] /= r; e[2] /= r;
+      const uint32_t idj = b.head_id[gh];
+      double dw[3];
+      adj_dw(w, idi, idj, dw);
+      const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
+      const double qq = (double)qi * (double)xq.w;
+      const double u1 = (pb.es_A * exp(-kap * r)) / r, k1 = kap + 1.0 / r;
+      const double e0 = u1 - pb.es_shiftU + pb.es_shiftF * (r - rc);
+      const double e1 = -u1 * k1 + pb.es_shiftF;
+      const double e2 = u1 * k1 * k1 + u1 / r2;
+      adj_pos(A, qq * e1, qq * e2, e, r, dw, ew);
+      if (idj > idi) {
+        A.dU[12] += 2.0 * qq * e0 / c.ph_q;
+        A.dF[12] -= ew * 2.0 * qq * e1 / c.ph_q;
+        A.dU[13] += qq * (-r * u1 + erc - kap * erc * (r - rc));
+        A.dF[13] -= ew * qq * (u1 * r * kap - kap * erc);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    A.g[a] = wave_sum_d_dpp(A.g[a]);
+    A.hw[a] = wave_sum_d_dpp(A.hw[a]);
+  }
+#pragma unroll
+  for (uint32_t p = 0; p < ADJ_P; ++p) {
+    A.dU[p] = wave_sum_d_dpp(A.dU[p]);
+    A.dF[p] = wave_sum_d_dpp(A.dF[p]);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g_out[3u * idi + a] = A.g[a];
+      if (hw_out) hw_out[3u * idi + a] = A.hw[a];
+    }
+#pragma unroll
+    for (uint32_t p = 0; p < ADJ_P; ++p) {
+      part[(size_t)t * (2u * ADJ_P) + p] = A.dU[p];
+      part[(size_t)t * (2u * ADJ_P) + ADJ_P + p] = A.dF[p];
+    }
+  }
+}
+
+// fixed-order sum of the per-particle parameter partials (k_potential_sum's scheme, 2 * ADJ_P values)
+__global__ __launch_bounds__(1024) void k_adjoint_sum(const double* part, uint32_t n, double* out) {
+  __shared__ double s[1024];
+  for (uint32_t p = 0; p < 2u * ADJ_P; ++p) {
+    double a = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a += part[(size_t)i * (2u * ADJ_P) + p];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[p] = s[0];
+    __syncthreads();
+  }
+}
+
+// ================================================================================================
 // soupPotential of an arbitrary configuration (soup/src/soup-potential.ts:145-301), for the area
 // move's trial boxes (soup/src/soup-area-move.ts:211-226): positions in particle order in their own
 // periodic box, with the system's bond slots, centre links and charges. The pairs come from the

@@ -56,6 +56,8 @@ struct SoupHandle {
   float4* pa_xs = nullptr;
   uint32_t *pa_cell = nullptr, *pa_order = nullptr, *pa_count = nullptr, *pa_start = nullptr;
   size_t pa_ncell_cap = 0;
+  // soup_energy_adjoint scratch (FP64): direction, gradient, Hessian-vector product, parameter partials
+  double *adj_w = nullptr, *adj_g = nullptr, *adj_hw = nullptr, *adj_part = nullptr, *adj_out = nullptr;
   SoupSweepResult last_sweep{};
   std::string err;
   std::vector<void*> allocs;
@@ -1019,6 +1021,50 @@ int soup_potential(SoupHandle* h, const SoupPotentialBasis* pb, double out[5]) {
   hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, part, h->n, dout);
   HC(h, hipMemcpyAsync(out, dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return sync(h);
+}
+
+int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const double* w, double* grad, double* hw,
+                        double* dU_dtheta, double* dFw_dtheta) {
+  if (!h) return SOUP_E_INVALID;
+  if (!pb || !grad) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: null basis or gradient output");
+  if ((hw || dFw_dtheta) && !w) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: the w-products need a direction w");
+  const double near_ok = (double)h->c.list_range - (double)h->c.skin;
+  if (pb->cutoff > near_ok + 1e-9)
+    return fail(h, SOUP_E_INVALID, fmt("soup_energy_adjoint: pair cutoff %.6f exceeds the Verlet list's guaranteed range %.6f",
+                                       pb->cutoff, near_ok));
+  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
+  if (es && pb->es_cutoff > (double)h->c.es_list_range - (double)h->c.skin + 1e-6)
+    return fail(h, SOUP_E_INVALID, fmt("soup_energy_adjoint: DH cutoff %.6f exceeds the DH list's guaranteed range %.6f",
+                                       pb->es_cutoff, (double)h->c.es_list_range - (double)h->c.skin));
+  if (es && h->c.ph_q == 0.0) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: chargeDeprotonated is 0");
+  int r = sync(h);
+  if (r) return r;
+  r = assert_lists(h);
+  if (r) return r;
+  const uint32_t n = h->n;
+  if (!h->adj_g) {
+    DA(h, h->adj_w, (size_t)n * 3);
+    DA(h, h->adj_g, (size_t)n * 3);
+    DA(h, h->adj_hw, (size_t)n * 3);
+    DA(h, h->adj_part, (size_t)n * 2 * ADJ_P);
+    DA(h, h->adj_out, 2 * ADJ_P);
+  }
+  if (w) HC(h, hipMemcpyAsync(h->adj_w, w, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (es) hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
+  hipLaunchKernelGGL(k_energy_adjoint, dim3(blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, es ? 1 : 0,
+                     w ? (const double*)h->adj_w : nullptr, h->adj_g, w ? h->adj_hw : nullptr, h->adj_part);
+  hipLaunchKernelGGL(k_adjoint_sum, dim3(1), dim3(1024), 0, h->stream, h->adj_part, n, h->adj_out);
+  double sums[2 * ADJ_P];
+  HC(h, hipMemcpyAsync(grad, h->adj_g, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (hw) HC(h, hipMemcpyAsync(hw, h->adj_hw, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HC(h, hipMemcpyAsync(sums, h->adj_out, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+  r = sync(h);
+  if (r) return r;
+  for (uint32_t p = 0; p < ADJ_P; ++p) {
+    if (dU_dtheta) dU_dtheta[p] = sums[p];
+    if (dFw_dtheta) dFw_dtheta[p] = sums[ADJ_P + p];
+  }
+  return 0;
 }
 
 int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3], double out[5]) {

@@ -98,7 +98,7 @@ SET_BOX_FORCE = 1  # soup_set_box: run a force pass after the rebuild
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
     "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_use_graphs", "soup_version",
 )
@@ -140,6 +140,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_potential_at": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),
         "soup_set_box": (C.c_int, [H, C.POINTER(SoupConfig), C.c_void_p, C.c_uint32]),
+        "soup_energy_adjoint": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
         "soup_proximity_pairs": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_void_p, C.c_void_p,
                                            C.c_uint64, C.POINTER(C.c_uint64)]),
         "soup_scale_molecules_rigid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_double),

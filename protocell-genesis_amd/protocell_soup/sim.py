@@ -442,6 +442,17 @@ class SoupSystem:
             self._area = SoupAreaMove(self)
         return self._area(trials, opts)
 
+    @property
+    def headCharge(self) -> float:
+        """chargeDeprotonated: the charge of a deprotonated head (every charge is 0 or this)."""
+        return float(self._R.cfg.ph_q)
+
+    def energyAdjoint(self, basis: Optional[native.SoupPotentialBasis] = None, w: Optional[np.ndarray] = None) -> Dict[str, Any]:
+        """Gradients of soupPotential at the current configuration (protocell_soup.adjoint)."""
+        from .adjoint import energyAdjoint
+
+        return energyAdjoint(self, basis, w)
+
     def potentialAt(self, positions: np.ndarray, box, basis: Optional[native.SoupPotentialBasis] = None) -> Dict[str, float]:
         """soupPotential of an arbitrary configuration of this system (positions in particle order, its
         own periodic box), with the system's bonds, centre links and charges, on the GPU."""
