@@ -186,6 +186,22 @@ const char* soup_last_error(const SoupHandle* h);
 /* SoupSystem.step(n) (soup/src/soup-integrate.ts:136-175): n integration steps in chunks of 1000,
  * with the non-finite and Verlet guards and the due constant-pH sweep at each chunk end. */
 int soup_step(SoupHandle* h, uint32_t n);
+/* n steps exactly as soup_step(h, n) advances them, with the state after every step copied out in
+ * particle order (each array nullable): pos[s * 4N ...], bonds[s * 3N ...], links[s * N ...],
+ * charges[s * N ...]; swept[s] = 1 where a constant-pH sweep ran after step s (its charges, and the
+ * force the next step starts from, are then the post-sweep ones). The forward record of a
+ * trajectory-segment adjoint. No counterpart in the reference. */
+int soup_step_record(SoupHandle* h, uint32_t n, float* pos, uint32_t* bonds, uint32_t* links, float* charges,
+                     uint8_t* swept);
+
+/* Replaces the interaction strengths of a live system (epsilon, the attraction table and acid-soap
+ * depth, the DH coefficient, kappa and shift force, thermostat and bond-rule rates) with those of
+ * *c; every geometric, range, radius and capacity field must equal the current one. The force the
+ * next step starts from is recomputed under the new parameters with the current lists. For
+ * parameter sweeps and finite-difference checks of the adjoint; no counterpart in the reference,
+ * whose parameters are fixed at creation. */
+int soup_set_params(SoupHandle* h, const SoupConfig* c);
+
 /* `count` systems advanced by n steps together (the replica sweep): every system's steps of a
  * 1,000-step chunk are queued first, each system on its own stream, round robin by step period,
  * then each system's chunk-end guards and due constant-pH sweep run. Every system ends exactly as

@@ -748,6 +748,67 @@ int soup_step(SoupHandle* h, uint32_t nsteps) {
   return 0;
 }
 
+int soup_step_record(SoupHandle* h, uint32_t nsteps, float* pos, uint32_t* bonds, uint32_t* links, float* charges,
+                     uint8_t* swept) {
+  if (!h) return SOUP_E_INVALID;
+  const size_t n = h->n;
+  uint32_t done = 0, s = 0;
+  while (done < nsteps) {
+    const uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
+    const uint64_t chunk_start = h->global_step;
+    for (uint32_t k = 0; k < chunk; ++k, ++s) {
+      one_step(h);
+      int r = 0;
+      if (pos) r = soup_read(h, SOUP_FIELD_POS, pos + (size_t)s * 4 * n, n * 16);
+      if (!r && bonds) r = soup_read(h, SOUP_FIELD_BOND_SLOTS, bonds + (size_t)s * 3 * n, n * 12);
+      if (!r && links) r = soup_read(h, SOUP_FIELD_CENTER_LINK, links + (size_t)s * n, n * 4);
+      if (!r && charges) r = soup_read(h, SOUP_FIELD_CHARGE, charges + (size_t)s * n, n * 4);
+      if (r) return r;
+      if (swept) swept[s] = 0;
+    }
+    const uint32_t sweeps0 = h->sweeps;
+    int r = chunk_end(h, chunk_start);
+    if (r) return r;
+    if (h->sweeps != sweeps0) {
+      // the sweep after this chunk's last step changed the charges and recomputed the force
+      if (swept) swept[s - 1] = 1;
+      if (charges) {
+        r = soup_read(h, SOUP_FIELD_CHARGE, charges + (size_t)(s - 1) * n, n * 4);
+        if (r) return r;
+      }
+    }
+    done += chunk;
+  }
+  return 0;
+}
+
+int soup_set_params(SoupHandle* h, const SoupConfig* c) {
+  if (!h) return SOUP_E_INVALID;
+  if (!c) return fail(h, SOUP_E_INVALID, "soup_set_params: null config");
+  const SoupConfig& o = h->c;
+  bool same = c->n == o.n && c->n_kinds == o.n_kinds && c->walk_radius == o.walk_radius && c->list_range == o.list_range &&
+              c->list_cap == o.list_cap && c->rebuild_every == o.rebuild_every && c->skin == o.skin &&
+              c->es_enabled == o.es_enabled && c->es_rc == o.es_rc && c->es_split == o.es_split &&
+              c->es_list_range == o.es_list_range && c->es_cap == o.es_cap && c->charged_kind == o.charged_kind &&
+              c->n_heads == o.n_heads && c->bond_interval == o.bond_interval && c->ph_q == o.ph_q;
+  for (int a = 0; a < 3; ++a) same = same && c->box[a] == o.box[a] && c->dims[a] == o.dims[a];
+  for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) same = same && c->radius[k] == o.radius[k] && c->cls[k] == o.cls[k];
+  same = same && c->sigma == o.sigma && c->wca_factor == o.wca_factor && c->b_tt == o.b_tt && c->wc == o.wc;
+  if (!same)
+    return fail(h, SOUP_E_INVALID,
+                "soup_set_params: only interaction strengths may change on a live system (epsilon, the attraction and acid-soap "
+                "depths, the DH coefficient, kappa and its shift force, the thermostat and bond rates); geometry, list ranges, "
+                "radii and capacities are fixed at creation");
+  int r = sync(h);
+  if (r) return r;
+  drop_graphs(h);  // their kernels carry the old parameters
+  h->c = *c;
+  // the force the next step starts from, under the new parameters (same lists: with unchanged
+  // parameters this reproduces the current force bit for bit)
+  ph_force(h);
+  return sync(h);
+}
+
 int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t nsteps, uint32_t* failed) {
   if (!hs || (count && !failed)) return SOUP_E_INVALID;
   for (uint32_t s = 0; s < count; ++s)

@@ -138,3 +138,101 @@ def __getattr__(name):
         e, f = _torch_functions()
         return e if name == "SoupEnergy" else f
     raise AttributeError(name)
+
+
+# ---- trajectory segments -------------------------------------------------------------------------
+def _topology(sys_, pos: np.ndarray, bonds: np.ndarray, links: np.ndarray, charges: np.ndarray) -> None:
+    """Loads one recorded state into a system and rebuilds its lists (positions, bond slots,
+    centre links, charges)."""
+    sys_._write("POS", np.ascontiguousarray(pos, dtype=np.float32).reshape(-1))
+    sys_._write("BOND_SLOTS", np.ascontiguousarray(bonds, dtype=np.uint32).reshape(-1))
+    sys_._write("CENTER_LINK", np.ascontiguousarray(links, dtype=np.uint32).reshape(-1))
+    sys_._write("CHARGE", np.ascontiguousarray(charges, dtype=np.float32).reshape(-1))
+    sys_._call(sys_._lib.soup_forces, native.FORCES_REBUILD)
+
+
+def _clone(sys_):
+    """A second handle holding the same state and parameters (for the reverse sweep's launches)."""
+    from .checkpoint import decodeCheckpointResume, encodeCheckpoint
+    from .sim import createSoup
+
+    R = sys_._R
+    opts = dict(R.opts)
+    opts["resume"] = decodeCheckpointResume(encodeCheckpoint(sys_, {}))
+    c = createSoup(opts)
+    c._call(c._lib.soup_set_params, C.byref(R.cfg))
+    return c
+
+
+def segmentGradient(sys_, steps: int, basis: Optional[native.SoupPotentialBasis] = None) -> Dict[str, Any]:
+    """Advances the system by `steps` steps, exactly as step(steps) would, and returns the gradient of
+    J = U(x_K; theta), the potential energy at the end, with respect to theta, to the start positions
+    x_0 and to the start velocities v_0, by one reverse sweep over the recorded states.
+
+    The step is the reference's velocity-Verlet Langevin step (step.wgsl:96-142): vh = v + dt/2 F(x),
+    x' = x + dt vh, v' = (1 - gamma dt)(vh + dt/2 F(x')) + noise. The noise does not depend on the
+    state, so the pathwise adjoint runs backward through the two half-kicks and the drift, with one
+    adjoint launch (-H w and d(w . F)/dtheta) per force evaluation. Bond and charge Monte Carlo are
+    not differentiated: each force is taken with the bond slots, centre links and charges it was
+    computed with, as recorded; a constant-pH sweep inside the segment splits the force a step
+    starts from from the one the previous step ended with, and both are honoured. The reverse
+    launches run on a second handle, so the system continues exactly as step(steps) leaves it.
+    The start must not follow a bond Monte Carlo step (the force buffer would hold the previous
+    topology's force)."""
+    R = sys_._R
+    cfg = R.cfg
+    n = sys_.N
+    g0 = sys_.steps
+    if g0 > 0 and (g0 - 1) % cfg.bond_interval == 0:
+        raise ValueError("segmentGradient: the segment starts right after a bond Monte Carlo step; start one step later")
+    if int((sys_.frozen() != 0).sum()) > 0:
+        raise ValueError("segmentGradient: frozen particles are not differentiated")
+    b = basis if basis is not None else sys_.potentialBasis()
+    K = int(steps)
+    x0 = sys_.particles().reshape(n, 4).copy()
+    B0, L0, C0 = sys_.bondSlots().reshape(n, 3).copy(), sys_.centerLinks().copy(), sys_.charges().copy()
+    P = np.empty((K, n, 4), np.float32)
+    Bs = np.empty((K, n, 3), np.uint32)
+    Ls = np.empty((K, n), np.uint32)
+    Cs = np.empty((K, n), np.float32)
+    sw = np.zeros(K, np.uint8)
+    sys_._call(sys_._lib.soup_step_record, C.c_uint32(K), P.ctypes.data_as(C.c_void_p), Bs.ctypes.data_as(C.c_void_p),
+               Ls.ctypes.data_as(C.c_void_p), Cs.ctypes.data_as(C.c_void_p), sw.ctypes.data_as(C.c_void_p))
+    J = sys_.potential(b)["total"]
+    end = energyAdjoint(sys_, b)
+    dt, a = float(cfg.dt), 1.0 - float(cfg.gamma) * float(cfg.dt)
+    clone = _clone(sys_)
+    try:
+        def start_of(s):  # topology at the start of step s
+            return (B0, L0, C0) if s == 0 else (Bs[s - 1], Ls[s - 1], Cs[s - 1])
+
+        def vjp(pos, topo, w):
+            _topology(clone, pos, *topo)
+            r = energyAdjoint(clone, b, w=w)
+            return -r["hw"], r["dFw"]
+
+        lx = end["grad"].copy()
+        lv = np.zeros((n, 3))
+        lth = end["dU"].copy()
+        lvh_next = np.zeros((n, 3))
+        for s in range(K - 1, -1, -1):
+            wA = a * (dt / 2) * lv                       # step s's second half-kick
+            wB = (dt / 2) * lvh_next if s < K - 1 else None  # step s + 1's first half-kick
+            if wB is not None and sw[s]:
+                gx, gt = vjp(P[s], (Bs[s], Ls[s], Cs[s]), wB)
+                lx += gx
+                lth += gt
+                wB = None
+            w = wA if wB is None else wA + wB
+            gx, gt = vjp(P[s], start_of(s), w)
+            lx += gx
+            lth += gt
+            lvh = a * lv + dt * lx
+            lv = lvh
+            lvh_next = lvh
+        gx, gt = vjp(x0, (B0, L0, C0), (dt / 2) * lvh_next)  # step 0's first half-kick
+        lx += gx
+        lth += gt
+    finally:
+        clone.dispose()
+    return {"J": J, "dJ_dtheta": lth, "dJ_dx0": lx, "dJ_dv0": lv, "steps": K, "sweeps": int(sw.sum())}

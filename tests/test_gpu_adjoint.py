@@ -140,7 +140,7 @@ def test_torch_energy_and_force_functions(soup):
         U = SoupEnergy.apply(x, th, s, b)
         U.backward()
         ref = s.energyAdjoint(b)
-        assert float(U) == pytest.approx(_U(s, b), rel=1e-14)
+        assert U.item() == pytest.approx(_U(s, b), rel=1e-14)
         assert np.array_equal(x.grad.numpy(), ref["grad"]) and np.array_equal(th.grad.numpy(), ref["dU"])
         # a loss on the forces: L = sum(c * F); dL/dx = -H c, dL/dtheta = d(c . F)/dtheta
         c = torch.tensor(np.random.default_rng(9).normal(size=x0.shape), dtype=torch.float64)
@@ -162,3 +162,65 @@ def test_torch_energy_and_force_functions(soup):
         assert (Lp - Lm) / (2 * h) == pytest.approx(float(th2.grad[k]), rel=1e-5)
     finally:
         setPositions(s, x0)
+
+
+def _segment_soup():
+    s = ps.createSoup(amphiphile_opts(16.0, 80, 4, {"W": 1800}, seed=6, electrostatics=CHARGED))
+    s.relaxColdStart()
+    s.step(65)  # not right after a bond Monte Carlo step (every 20)
+    return s
+
+
+def test_segment_gradient_against_perturbed_reruns():
+    import ctypes as C
+
+    from protocell_soup import native
+    from protocell_soup.adjoint import segmentGradient
+
+    K = 20  # one bond Monte Carlo pass inside
+    s = _segment_soup()
+    try:
+        b = s.potentialBasis()
+        res = segmentGradient(s, K, b)
+        assert s.steps == 65 + K
+        bonds_end = s.bondSlots()
+        pos_end = s.particles()
+    finally:
+        s.dispose()
+    # the recording run is the plain trajectory: the same end state as step(K)
+    t = _segment_soup()
+    try:
+        t.step(K)
+        assert np.array_equal(t.particles(), pos_end)
+        assert t.potential(b)["total"] == res["J"]
+    finally:
+        t.dispose()
+    # d J / d theta against reruns of the device dynamics with the parameter moved by +-h (a power
+    # of two, so the f32 value on the device moves by exactly h). The reruns integrate in f32, whose
+    # rounding the dynamics amplify (measured at 40 steps: 5e-4 on the tail pair, 3e-5 on the water
+    # pair); at 20 steps and h = 2^-8 that noise and the truncation stay near 1e-5
+    h = 2.0 ** -8
+    for k in (PARAM_NAMES.index("attr22"), PARAM_NAMES.index("attr00"), PARAM_NAMES.index("epsilon")):
+        J = []
+        for sign in (1.0, -1.0):
+            r = _segment_soup()
+            try:
+                cfg = native.SoupConfig()
+                C.memmove(C.byref(cfg), C.byref(r._R.cfg), C.sizeof(native.SoupConfig))
+                th = paramsOf(b, r.headCharge)
+                th[k] += sign * h
+                if k == PARAM_NAMES.index("epsilon"):
+                    cfg.epsilon = cfg.epsilon + sign * h
+                else:
+                    a_, c_ = [(x, y) for x in range(4) for y in range(x, 4)][k]
+                    cfg.attr[a_][c_] = cfg.attr[a_][c_] + sign * h
+                    cfg.attr[c_][a_] = cfg.attr[a_][c_]
+                r._call(r._lib.soup_set_params, C.byref(cfg))
+                r.step(K)
+                assert np.array_equal(r.bondSlots(), bonds_end)  # no Monte Carlo event moved
+                J.append(r.potential(basisWith(b, th, r.headCharge))["total"])
+            finally:
+                r.dispose()
+        fd = (J[0] - J[1]) / (2 * h)
+        print(PARAM_NAMES[k], "adjoint", res["dJ_dtheta"][k], "differences", fd)
+        assert fd == pytest.approx(res["dJ_dtheta"][k], rel=1e-4), PARAM_NAMES[k]
