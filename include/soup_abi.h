@@ -88,7 +88,8 @@ typedef struct SoupConfig {
   float max_hold_steps;       /* adsorptionParams.x */
   float bond_interval_f;      /* adsorptionParams.y */
   uint32_t bond_interval;     /* bondAttemptInterval.steps */
-  uint32_t reserved[7];
+  uint32_t capacity;          /* particles the buffers hold (the creation census; 0: n) */
+  uint32_t reserved[6];
 } SoupConfig;
 
 /* Mutable state at creation: a fresh lattice built by the host (soup/src/soup-init-state.ts:58-307)
@@ -194,6 +195,22 @@ int soup_step(SoupHandle* h, uint32_t n);
 int soup_step_record(SoupHandle* h, uint32_t n, float* pos, uint32_t* bonds, uint32_t* links, float* charges,
                      uint8_t* swept);
 
+/* setActiveCount (soup/src/soup-evaporate.ts:58-80) with the state it needs: replaces every
+ * per-particle array with st's for n particles (particle order; n <= the capacity), keeps the step
+ * counter, event counters and the constant-pH stream, and runs a rebuild and a force pass. Solvent
+ * evaporation passes the first n particles of the current state, rehydration the current state
+ * followed by the returning beads. Charged heads must keep their indices. */
+int soup_reset_particles(SoupHandle* h, uint32_t n, const SoupInitState* st);
+
+/* sampleInsertionPositions (soup/src/soup-evaporate-plan.ts:202-304), host code: `count` positions
+ * in `box`, each the best of up to 64 uniform candidates by distance to everything already there
+ * (the active particles, then the beads placed before it), stopping at the first candidate at least
+ * min_sep away. Candidates come from a mulberry32 stream whose state *rng is advanced in place.
+ * out: count x (x, y, z, 0) in f32; *short_of_floor: beads placed below min_sep; *min_achieved: the
+ * smallest such distance (0 if none was placed). */
+int soup_sample_insertion(const float* active, uint32_t n_active, const double box[3], uint32_t count, double min_sep,
+                          uint32_t* rng, float* out, uint32_t* short_of_floor, double* min_achieved);
+
 /* Replaces the interaction strengths of a live system (epsilon, the attraction table and acid-soap
  * depth, the DH coefficient, kappa and shift force, thermostat and bond-rule rates) with those of
  * *c; every geometric, range, radius and capacity field must equal the current one. The force the
@@ -214,7 +231,8 @@ int soup_global_step(const SoupHandle* h, uint64_t* out);
 
 /* Readbacks (soup/src/soup-readback.ts) and writes; bytes must equal the field size. */
 int soup_read(SoupHandle* h, int field, void* dst, size_t bytes);
-/* Writable: POS, VEL, CHARGE, BOND_SLOTS, CENTER_LINK, CENTER_HELD, BOND_RNG, THERMO_RNG. */
+/* Writable: POS, VEL, CHARGE, BOND_SLOTS, CENTER_LINK, CENTER_HELD, BOND_RNG, THERMO_RNG, DESORB (the two
+ * desorption counters, which desorbOverstretchedTethers adds to: soup/src/soup-evaporate.ts:316-325). */
 int soup_write(SoupHandle* h, int field, const void* src, size_t bytes);
 /* Size in bytes of a field (0 for an unknown field). */
 size_t soup_field_bytes(const SoupHandle* h, int field);
@@ -379,7 +397,7 @@ int soup_analysis_cavity(SoupAnalysis* a, const float* pos4, uint32_t m, const d
 
 /* relaxIterations (soup/src/soup-relax.ts:87-139): normalised steepest descent with the cap
  * decaying linearly from d0 to 0, a rebuild before every iteration, and a final rebuild + force. */
-int soup_relax(SoupHandle* h, uint32_t iterations, float d0);
+int soup_relax(SoupHandle* h, uint32_t iterations, double d0);
 
 /* Device-time profile per phase, measured with HIP events on the handle's own stream around the
  * launches of each phase. soup_profile(h, mask) resets the totals and brackets the phases whose bit

@@ -49,7 +49,7 @@ def lib() -> C.CDLL:
         _lib.or_es_build.restype = C.c_uint32
         _lib.or_threads.restype = C.c_int
         _lib.or_relax_step.argtypes = [C.c_void_p, C.c_void_p, C.c_float]
-        _lib.or_relax.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float]
+        _lib.or_relax.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_double]
     return _lib
 
 
@@ -175,7 +175,7 @@ class OracleState:
         return int(lib().or_step(self._c, self._s, C.c_uint32(nsteps), C.byref(self.last_sweep), C.byref(self.sweeps)))
 
     def relax(self, iterations: int, d0: float) -> int:
-        return int(lib().or_relax(self._c, self._s, C.c_uint32(iterations), C.c_float(d0)))
+        return int(lib().or_relax(self._c, self._s, C.c_uint32(iterations), C.c_double(d0)))
 
     def nonfinite(self):
         out = (C.c_uint32 * 2)()

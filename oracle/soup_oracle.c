@@ -1458,9 +1458,9 @@ int or_step(const SoupConfig* c, OrState* s, uint32_t nsteps, SoupSweepResult* l
   return OR_OK;
 }
 
-int or_relax(const SoupConfig* c, OrState* s, uint32_t iterations, float d0) {
+int or_relax(const SoupConfig* c, OrState* s, uint32_t iterations, double d0) {
   for (uint32_t k = 0; k < iterations; ++k) {
-    float d = (float)((double)d0 * (1.0 - (double)k / (double)iterations));
+    float d = (float)(d0 * (1.0 - (double)k / (double)iterations));
     or_rebuild(c, s);
     or_force_list(c, s);
     or_relax_step(c, s, d);

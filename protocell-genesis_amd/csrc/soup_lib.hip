@@ -37,6 +37,8 @@ struct Prof {
 struct SoupHandle {
   SoupConfig c;
   uint32_t n = 0, ncells = 0, nh = 0;
+  uint32_t cap = 0;              // particles every per-particle buffer holds (n <= cap)
+  std::vector<uint32_t> heads;   // particle indices of the charged heads, ascending
   hipStream_t stream = nullptr;
   DevBuf b{};
   DevBuf alt{};
@@ -479,8 +481,14 @@ static int validate(const SoupConfig* c) {
   if (c->walk_radius > 3)
     return fail(nullptr, SOUP_E_INVALID,
                 fmt("soup_create: list walk radius %u exceeds 3 (the list build sets up one cell row per lane)", c->walk_radius));
-  if (c->n >= (1u << 29))
-    return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: N=%u exceeds the 2^29 - 1 particles the kernels address", c->n));
+  if (c->n >= (1u << 29) || c->capacity >= (1u << 29))
+    return fail(nullptr, SOUP_E_INVALID,
+                fmt("soup_create: N=%u (capacity %u) exceeds the 2^29 - 1 particles the kernels address", c->n, c->capacity));
+  if (c->capacity != 0 && c->capacity < c->n)
+    return fail(nullptr, SOUP_E_INVALID,
+                fmt("createSoup: the live composition (%u) is larger than the starting one (%u) -- buffers are allocated for the "
+                    "starting one",
+                    c->n, c->capacity));
   if (c->list_cap == 0 || c->rebuild_every == 0 || c->bond_interval == 0)
     return fail(nullptr, SOUP_E_INVALID, "soup_create: list capacity, rebuildEvery and bondAttemptInterval must be > 0");
   if (c->es_enabled && c->n_heads > 0 && c->es_cap == 0)
@@ -491,45 +499,86 @@ static int validate(const SoupConfig* c) {
   return 0;
 }
 
+// The initial-state arrays for n particles: every index the kernels follow must be in range before
+// anything reaches the device (h is null while soup_create has no handle yet)
+static int check_state(SoupHandle* h, const SoupConfig& c, uint32_t n, const SoupInitState* st, const char* who) {
+  if (!st || !st->positions || !st->velocities || !st->bond_slots || !st->center_link || !st->center_held || !st->frozen ||
+      !st->bond_rng || !st->thermo_rng || !st->charges)
+    return fail(h, SOUP_E_INVALID, fmt("%s: every initial-state array is required", who));
+  for (uint32_t i = 0; i < n; ++i) {
+    float w = st->positions[4u * i + 3];
+    if (!(w >= 0.0f) || w != std::floor(w) || w >= (float)c.n_kinds)
+      return fail(h, SOUP_E_INVALID, fmt("%s: particle %u has kind %g outside the %u monomer kinds", who, i, w, c.n_kinds));
+    for (int k = 0; k < 3; ++k) {
+      uint32_t p = st->bond_slots[3u * i + k];
+      if (p != SOUP_NONE && (p >= n || p == i))
+        return fail(h, SOUP_E_INVALID, fmt("%s: bond slot %d of particle %u names %u (N=%u)", who, k, i, p, n));
+    }
+    uint32_t l = st->center_link[i];
+    if (l != SOUP_NONE && (l >= n || l == i))
+      return fail(h, SOUP_E_INVALID, fmt("%s: centerLink of particle %u names %u (N=%u)", who, i, l, n));
+    const float q = st->charges[i];
+    if (q != 0.0f && q != (float)c.ph_q)
+      return fail(h, SOUP_E_INVALID,
+                  fmt("%s: charge of particle %u is %g; charges are 0 or chargeDeprotonated=%g", who, i, (double)q, c.ph_q));
+  }
+  return 0;
+}
+
+// ascending particle indices of the charged kind (kinds never change, so this is the compaction
+// soup_build_head_index_main recomputes at every rebuild)
+static std::vector<uint32_t> heads_of(const SoupConfig& c, uint32_t n, const float* positions) {
+  std::vector<uint32_t> heads;
+  if (c.es_enabled && c.n_heads > 0)
+    for (uint32_t i = 0; i < n; ++i)
+      if (positions[4u * i + 3] == (float)c.charged_kind) heads.push_back(i);
+  return heads;
+}
+
+// Writes n particles' state into the current buffers with slot = particle index (the identity
+// order the next rebuild sorts from); the force is cleared for the priming pass to fill.
+static int upload_state(SoupHandle* h, uint32_t n, const SoupInitState* st) {
+  DevBuf& b = h->b;
+  std::vector<uint4> bs(n);
+  for (uint32_t i = 0; i < n; ++i)
+    bs[i] = make_uint4(st->bond_slots[3u * i], st->bond_slots[3u * i + 1], st->bond_slots[3u * i + 2], 0u);
+  std::vector<uint32_t> ids(n);
+  for (uint32_t i = 0; i < n; ++i) ids[i] = i;
+  HC(h, hipMemcpy(b.pos, st->positions, (size_t)n * 16, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.vel, st->velocities, (size_t)n * 16, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.charge, st->charges, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.bs, bs.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.clink, st->center_link, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.cheld, st->center_held, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.frozen, st->frozen, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.brng, st->bond_rng, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.trng, st->thermo_rng, (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemcpy(b.inv, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  HC(h, hipMemset(b.force, 0, (size_t)h->cap * 16));
+  HC(h, hipMemset(h->alt.force, 0, (size_t)h->cap * 16));
+  HC(h, hipMemset(b.resv, 0xFF, (size_t)h->cap * 4));
+  return 0;
+}
+
 int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** out) {
   if (!out) return fail(nullptr, SOUP_E_INVALID, "soup_create: null output handle");
   *out = nullptr;
   int v = validate(cfg);
   if (v) return v;
-  if (!init || !init->positions || !init->velocities || !init->bond_slots || !init->center_link || !init->center_held ||
-      !init->frozen || !init->bond_rng || !init->thermo_rng || !init->charges)
-    return fail(nullptr, SOUP_E_INVALID, "soup_create: every initial-state array is required");
-  // every index the kernels follow must be in range before anything reaches the device
-  for (uint32_t i = 0; i < cfg->n; ++i) {
-    float w = init->positions[4u * i + 3];
-    if (!(w >= 0.0f) || w != std::floor(w) || w >= (float)cfg->n_kinds)
-      return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: particle %u has kind %g outside the %u monomer kinds", i, w, cfg->n_kinds));
-    for (int k = 0; k < 3; ++k) {
-      uint32_t p = init->bond_slots[3u * i + k];
-      if (p != SOUP_NONE && (p >= cfg->n || p == i))
-        return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: bond slot %d of particle %u names %u (N=%u)", k, i, p, cfg->n));
-    }
-    uint32_t l = init->center_link[i];
-    if (l != SOUP_NONE && (l >= cfg->n || l == i))
-      return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: centerLink of particle %u names %u (N=%u)", i, l, cfg->n));
-    const float q = init->charges[i];
-    if (q != 0.0f && q != (float)cfg->ph_q)
-      return fail(nullptr, SOUP_E_INVALID,
-                  fmt("soup_create: charge of particle %u is %g; charges are 0 or chargeDeprotonated=%g", i, (double)q, cfg->ph_q));
-  }
+  v = check_state(nullptr, *cfg, cfg->n, init, "soup_create");
+  if (v) return v;
   SoupHandle* h = new SoupHandle();
   h->c = *cfg;
   h->n = cfg->n;
+  h->cap = std::max(cfg->n, cfg->capacity);
+  h->c.capacity = h->cap;
   h->ncells = cfg->dims[0] * cfg->dims[1] * cfg->dims[2];
-  const uint32_t n = h->n;
-  // heads: ascending particle indices of the charged kind; kinds never change, so this is the
-  // compaction soup_build_head_index_main recomputes at every rebuild
-  std::vector<uint32_t> heads;
-  if (cfg->es_enabled && cfg->n_heads > 0) {
-    for (uint32_t i = 0; i < n; ++i)
-      if (init->positions[4u * i + 3] == (float)cfg->charged_kind) heads.push_back(i);
-  }
-  h->nh = (uint32_t)heads.size();
+  // every per-particle buffer holds the capacity, so a smaller live count can grow back
+  const uint32_t n = h->cap;
+  h->heads = heads_of(*cfg, cfg->n, init->positions);
+  h->nh = (uint32_t)h->heads.size();
+  const std::vector<uint32_t>& heads = h->heads;
   int r = 0;
   auto cleanup = [&](int code) {
     std::string e = h->err;
@@ -615,25 +664,8 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_q = b.head_q;
         a.far_acc = b.far_acc;
         a.gq = b.gq;
-        // initial state: slot = particle index
-        std::vector<uint4> bs(n);
-        for (uint32_t i = 0; i < n; ++i)
-          bs[i] = make_uint4(init->bond_slots[3u * i], init->bond_slots[3u * i + 1], init->bond_slots[3u * i + 2], 0u);
-        std::vector<uint32_t> ids(n);
-        for (uint32_t i = 0; i < n; ++i) ids[i] = i;
-        HC(h, hipMemcpy(b.pos, init->positions, (size_t)n * 16, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.vel, init->velocities, (size_t)n * 16, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.charge, init->charges, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.bs, bs.data(), (size_t)n * 16, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.clink, init->center_link, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.cheld, init->center_held, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.frozen, init->frozen, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.brng, init->bond_rng, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.trng, init->thermo_rng, (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.id, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemcpy(b.inv, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-        HC(h, hipMemset(b.force, 0, (size_t)n * 16));
-        HC(h, hipMemset(a.force, 0, (size_t)n * 16));
+        int u = upload_state(h, h->n, init);
+        if (u) return u;
         HC(h, hipMemset(b.flags, 0, 40));
         uint32_t ev[2 * SOUP_MAX_RULES] = {0};
         if (init->events)
@@ -645,7 +677,6 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
           de[1] = init->desorb_events[1];
         }
         HC(h, hipMemcpy(b.desorb, de, 8, hipMemcpyHostToDevice));
-        HC(h, hipMemset(b.resv, 0xFF, (size_t)n * 4));
         if (h->nh > 0) HC(h, hipMemcpy(b.head_id, heads.data(), (size_t)h->nh * 4, hipMemcpyHostToDevice));
         size_t lds = (size_t)h->nh * sizeof(float);
         if (lds > 64 * 1024) HC(h, hipFuncSetAttribute((const void*)k_proton_chain, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -660,6 +691,36 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
     return cleanup(r);
   *out = h;
   return 0;
+}
+
+int soup_reset_particles(SoupHandle* h, uint32_t n, const SoupInitState* st) {
+  if (!h) return SOUP_E_INVALID;
+  if (n == 0 || n > h->cap)
+    return fail(h, SOUP_E_INVALID,
+                fmt("soup_reset_particles: %u particles requested, the buffers hold 1..%u (the creation census)", n, h->cap));
+  int r = check_state(h, h->c, n, st, "soup_reset_particles");
+  if (r) return r;
+  if (heads_of(h->c, n, st->positions) != h->heads)
+    return fail(h, SOUP_E_INVALID,
+                fmt("soup_reset_particles: the charged heads must keep their particle indices (the long-range list and the "
+                    "constant-pH chain are sized and indexed for the %u heads of the creation census)",
+                    h->nh));
+  r = sync(h);
+  if (r) return r;
+  drop_graphs(h);  // captured for the old particle count
+  r = upload_state(h, n, st);
+  if (r) return r;
+  h->n = n;
+  h->c.n = n;
+  // setActiveCount's post-conditions (soup/src/soup-evaporate.ts:58-80): lists and force of the new
+  // state, then the finiteness and list guards
+  ph_rebuild(h);
+  ph_force(h);
+  r = sync(h);
+  if (r) return r;
+  r = assert_finite(h, false, 0);
+  if (r) return r;
+  return assert_lists(h);
 }
 
 int soup_use_graphs(SoupHandle* h, int enable) {
@@ -790,7 +851,8 @@ int soup_set_params(SoupHandle* h, const SoupConfig* c) {
               c->list_cap == o.list_cap && c->rebuild_every == o.rebuild_every && c->skin == o.skin &&
               c->es_enabled == o.es_enabled && c->es_rc == o.es_rc && c->es_split == o.es_split &&
               c->es_list_range == o.es_list_range && c->es_cap == o.es_cap && c->charged_kind == o.charged_kind &&
-              c->n_heads == o.n_heads && c->bond_interval == o.bond_interval && c->ph_q == o.ph_q;
+              c->n_heads == o.n_heads && c->bond_interval == o.bond_interval && c->ph_q == o.ph_q &&
+              std::max(c->n, c->capacity) == o.capacity;
   for (int a = 0; a < 3; ++a) same = same && c->box[a] == o.box[a] && c->dims[a] == o.dims[a];
   for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) same = same && c->radius[k] == o.radius[k] && c->cls[k] == o.cls[k];
   same = same && c->sigma == o.sigma && c->wca_factor == o.wca_factor && c->b_tt == o.b_tt && c->wc == o.wc;
@@ -965,6 +1027,7 @@ int soup_write(SoupHandle* h, int field, const void* srcp, size_t bytes) {
     case SOUP_FIELD_BOND_RNG: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.brng, n); break;
     case SOUP_FIELD_THERMO_RNG: hipLaunchKernelGGL(k_from_index<uint32_t>, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.trng, n); break;
     case SOUP_FIELD_BOND_SLOTS: hipLaunchKernelGGL(k_bs_from_index, g, bl, 0, h->stream, (const uint32_t*)h->stage, h->b.id, h->b.bs, n); break;
+    case SOUP_FIELD_DESORB: HC(h, hipMemcpyAsync(h->b.desorb, h->stage, 8, hipMemcpyDeviceToDevice, h->stream)); break;
     default: return fail(h, SOUP_E_INVALID, fmt("soup_write: field %d is not writable", field));
   }
   return sync(h);
@@ -1103,11 +1166,11 @@ int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const doubl
   r = assert_lists(h);
   if (r) return r;
   const uint32_t n = h->n;
-  if (!h->adj_g) {
-    DA(h, h->adj_w, (size_t)n * 3);
-    DA(h, h->adj_g, (size_t)n * 3);
-    DA(h, h->adj_hw, (size_t)n * 3);
-    DA(h, h->adj_part, (size_t)n * 2 * ADJ_P);
+  if (!h->adj_g) {  // sized for the capacity: the live count may grow back after an evaporation
+    DA(h, h->adj_w, (size_t)h->cap * 3);
+    DA(h, h->adj_g, (size_t)h->cap * 3);
+    DA(h, h->adj_hw, (size_t)h->cap * 3);
+    DA(h, h->adj_part, (size_t)h->cap * 2 * ADJ_P);
     DA(h, h->adj_out, 2 * ADJ_P);
   }
   if (w) HC(h, hipMemcpyAsync(h->adj_w, w, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -1154,9 +1217,9 @@ int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* 
   int r = sync(h);
   if (r) return r;
   if (!h->pa_xs) {
-    DA(h, h->pa_xs, n);
-    DA(h, h->pa_cell, n);
-    DA(h, h->pa_order, n);
+    DA(h, h->pa_xs, h->cap);
+    DA(h, h->pa_cell, h->cap);
+    DA(h, h->pa_order, h->cap);
   }
   size_t cap_c = h->pa_ncell_cap;
   r = dgrow(h, &h->pa_count, ncell + 1, cap_c);
@@ -1367,10 +1430,120 @@ int soup_proximity_pairs(const float* positions, uint32_t n, const double box[3]
   return 0;
 }
 
-int soup_relax(SoupHandle* h, uint32_t iterations, float d0) {
+// mulberry32 (soup/src/soup-evaporate.ts:169-174): advances the state and returns the draw in [0, 1)
+static double mulberry32_next(uint32_t& a) {
+  a += 0x6d2b79f5u;
+  uint32_t t = (a ^ (a >> 15)) * (1u | a);
+  t = (t + (t ^ (t >> 7)) * (61u | t)) ^ t;
+  return (double)(t ^ (t >> 14)) / 4294967296.0;
+}
+
+// sampleInsertionPositions (soup/src/soup-evaporate-plan.ts:202-304), host code in FP64 as the
+// reference computes it: cells of side >= min_sep (floor(box / min_sep) per axis) keyed
+// cx + nx (cy + ny cz), the 27 wrapped neighbour cells searched for the nearest point under the
+// minimum image, candidates x, y, z drawn in that order. The search only ever looks up the keys of
+// wrapped cells, so a point whose own key falls outside them (a coordinate below 0) is never found,
+// as in the reference; placed beads join the search at their FP64 coordinates.
+int soup_sample_insertion(const float* active, uint32_t n_active, const double box[3], uint32_t count, double min_sep,
+                          uint32_t* rng, float* out, uint32_t* short_of_floor, double* min_achieved) {
+  if (!box || !rng || (count && !out) || !short_of_floor || !min_achieved || (n_active && !active) ||
+      !(min_sep > 0.0) || !std::isfinite(min_sep))
+    return SOUP_E_INVALID;
+  int64_t nc[3];
+  double w[3];
+  for (int a = 0; a < 3; ++a) {
+    if (!(box[a] > 0.0) || !std::isfinite(box[a])) return SOUP_E_INVALID;
+    nc[a] = std::max<int64_t>(1, (int64_t)std::floor(box[a] / min_sep));
+    w[a] = box[a] / (double)nc[a];
+  }
+  const int64_t ncell = nc[0] * nc[1] * nc[2];
+  if (ncell > (int64_t)1 << 27) return SOUP_E_INVALID;
+  for (size_t i = 0; i < (size_t)n_active * 4; ++i)
+    if (!std::isfinite(active[i])) return SOUP_E_INVALID;
+  // per-cell singly linked lists: head[key] -> last point pushed, link[p] -> the one before
+  std::vector<int64_t> head((size_t)ncell, -1);
+  std::vector<int64_t> link;
+  std::vector<double> xs, ys, zs;
+  const size_t total = (size_t)n_active + count;
+  link.reserve(total);
+  xs.reserve(total);
+  ys.reserve(total);
+  zs.reserve(total);
+  auto cell = [&](double v, int a) { return std::min<int64_t>(nc[a] - 1, (int64_t)std::floor(v / w[a])); };
+  auto push = [&](double x, double y, double z) {
+    const int64_t idx = (int64_t)xs.size();
+    xs.push_back(x);
+    ys.push_back(y);
+    zs.push_back(z);
+    const int64_t key = cell(x, 0) + nc[0] * (cell(y, 1) + nc[1] * cell(z, 2));
+    if (key >= 0 && key < ncell) {
+      link.push_back(head[(size_t)key]);
+      head[(size_t)key] = idx;
+    } else {
+      link.push_back(-1);
+    }
+  };
+  for (uint32_t i = 0; i < n_active; ++i)
+    push((double)active[4u * i], (double)active[4u * i + 1], (double)active[4u * i + 2]);
+  auto mi1 = [](double d, double L) { return d - host_js_round(d / L) * L; };
+  auto nearest = [&](double x, double y, double z) {
+    const int64_t cx = cell(x, 0), cy = cell(y, 1), cz = cell(z, 2);
+    double best = INFINITY;
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int64_t ax = ((cx + dx) % nc[0] + nc[0]) % nc[0];
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int64_t ay = ((cy + dy) % nc[1] + nc[1]) % nc[1];
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int64_t az = ((cz + dz) % nc[2] + nc[2]) % nc[2];
+          for (int64_t j = head[(size_t)(ax + nc[0] * (ay + nc[1] * az))]; j >= 0; j = link[(size_t)j]) {
+            const double mx = mi1(x - xs[(size_t)j], box[0]);
+            const double my = mi1(y - ys[(size_t)j], box[1]);
+            const double mz = mi1(z - zs[(size_t)j], box[2]);
+            const double d2 = mx * mx + my * my + mz * mz;
+            if (d2 < best) best = d2;
+          }
+        }
+      }
+    }
+    return std::sqrt(best);
+  };
+  uint32_t shortc = 0;
+  double min_a = INFINITY;
+  uint32_t a = *rng;
+  for (uint32_t k = 0; k < count; ++k) {
+    double bx = 0.0, by = 0.0, bz = 0.0, bd = -1.0;
+    for (int t = 0; t < 64; ++t) {  // INSERTION_CANDIDATES
+      const double x = mulberry32_next(a) * box[0];
+      const double y = mulberry32_next(a) * box[1];
+      const double z = mulberry32_next(a) * box[2];
+      const double d = nearest(x, y, z);
+      if (d > bd) {
+        bd = d;
+        bx = x;
+        by = y;
+        bz = z;
+      }
+      if (d >= min_sep) break;
+    }
+    if (bd < min_sep) ++shortc;
+    if (bd < min_a) min_a = bd;
+    out[4u * k] = (float)bx;
+    out[4u * k + 1] = (float)by;
+    out[4u * k + 2] = (float)bz;
+    out[4u * k + 3] = 0.0f;
+    push(bx, by, bz);
+  }
+  *rng = a;
+  *short_of_floor = shortc;
+  *min_achieved = std::isfinite(min_a) ? min_a : 0.0;
+  return 0;
+}
+
+int soup_relax(SoupHandle* h, uint32_t iterations, double d0) {
   if (!h) return SOUP_E_INVALID;
   for (uint32_t k = 0; k < iterations; ++k) {
-    float d = (float)((double)d0 * (1.0 - (double)k / (double)iterations));
+    // the cap in FP64, rounded once to f32 (soup-relax.ts:92-93)
+    float d = (float)(d0 * (1.0 - (double)k / (double)iterations));
     ph_rebuild(h);
     ph_force(h);
     hipLaunchKernelGGL(k_relax_step, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b, d);

@@ -76,8 +76,10 @@ def computeDryBox(box: Sequence[float], N: int, targetDryDensity: float) -> Box:
 
 
 def deriveCycleConfig(soup: Dict[str, Any], opts: Dict[str, Any], box: Sequence[float], N: int,
-                      startCounts: Dict[str, int]) -> Tuple[Optional[CycleSchedule], Box]:
-    """soup-box-scale-math.ts:316-361 for unconfined systems without evaporation."""
+                      startCounts: Dict[str, int], dryBoxOverride: Optional[Sequence[float]] = None
+                      ) -> Tuple[Optional[CycleSchedule], Box]:
+    """soup-box-scale-math.ts:316-361 for unconfined systems; ``dryBoxOverride`` is the dry box of an
+    evaporating cycle (sized for what is left, planEvaporation), None the box for the census N."""
     enabled = opts.get("dryWetCycle", soup["dryWetCycle"]["enabled"])
     box = tuple(float(x) for x in box)
     if not enabled:
@@ -88,7 +90,7 @@ def deriveCycleConfig(soup: Dict[str, Any], opts: Dict[str, Any], box: Sequence[
         raise ValueError(
             f"data/soup.json: dryWetCycle.targetDryDensity={_js_num(dwc['targetDryDensity'])} does not exceed the current broth "
             f"density {wet_density:.4f} (N={N}, box=[{','.join(_js_num(x) for x in box)}]) -- the dry phase must concentrate, not dilute")
-    dry_box = computeDryBox(box, N, dwc["targetDryDensity"])
+    dry_box = tuple(float(x) for x in dryBoxOverride) if dryBoxOverride is not None else computeDryBox(box, N, dwc["targetDryDensity"])
     dry_plan = planSoupGrid(dry_box, startCounts)
     if not dry_plan.valid:
         raise ValueError(dry_plan.reason)
@@ -233,7 +235,8 @@ def growBoxTo(sys_, targetBox: Sequence[float], rampSteps: int, rampRelaxSteps: 
 
 def stepCycled(sys_, n: int) -> None:
     """makeStepCycled (soup-box-scale.ts:478-532): n steps through the dry-wet schedule; at each
-    transition the box ramps to the dry box or back to the wet (creation) box."""
+    transition the box ramps to the dry box or back to the wet (creation) box, and on an evaporating
+    system the solvent leaves or returns on the way (protocell_soup.evaporate)."""
     R = sys_._R
     cfg = R.cycle_cfg
     if cfg is None:
@@ -249,7 +252,15 @@ def stepCycled(sys_, n: int) -> None:
             remaining -= int(advance)
         if nxt != math.inf and sys_.steps == nxt:
             ph = cyclePhaseAt(sys_.steps, cfg)
-            applyBoxScale(sys_, R.dry_box if ph["phase"] == "dry" else R.box, dwc["rampSteps"], dwc["rampRelaxSteps"])
+            if getattr(R, "evap", None) is not None:
+                # the solvent leaves and returns at the same transition points
+                from .evaporate import applyEvaporatingTransition
+
+                rungs: list = []
+                reports = applyEvaporatingTransition(sys_, R.evap, ph["phase"], log=rungs.append)
+                R.last_transition = {"phase": ph["phase"], "step": int(nxt), "rungs": rungs, "rehydration": reports}
+            else:
+                applyBoxScale(sys_, R.dry_box if ph["phase"] == "dry" else R.box, dwc["rampSteps"], dwc["rampRelaxSteps"])
             R.cycle_phase = ph["phase"]
             R.cycle_index = ph["cycleIndex"]
         elif advance == 0 and remaining > 0:

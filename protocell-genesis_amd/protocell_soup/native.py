@@ -49,7 +49,7 @@ class SoupConfig(C.Structure):
         ("roleB", C.c_float * MAX_RULES),
         ("catalyst_kind", C.c_float), ("head_terminal_only", C.c_float), ("head_chain_cap", C.c_float),
         ("max_hold_steps", C.c_float), ("bond_interval_f", C.c_float), ("bond_interval", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("capacity", C.c_uint32), ("reserved", C.c_uint32 * 6),
     ]
 
 
@@ -96,7 +96,7 @@ SET_BOX_FORCE = 1  # soup_set_box: run a force pass after the rebuild
 
 # every entry point include/soup_abi.h declares
 EXPORTS = (
-    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_global_step", "soup_read", "soup_write",
+    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_reset_particles", "soup_sample_insertion", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
     "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
@@ -125,6 +125,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_step_batch": (C.c_int, [C.POINTER(H), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
         "soup_step_record": (C.c_int, [H, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "soup_set_params": (C.c_int, [H, C.POINTER(SoupConfig)]),
+        "soup_reset_particles": (C.c_int, [H, C.c_uint32, C.POINTER(SoupInitState)]),
+        "soup_sample_insertion": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_uint32, C.c_double,
+                                            C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
         "soup_global_step": (C.c_int, [H, C.POINTER(C.c_uint64)]),
         "soup_read": (C.c_int, [H, C.c_int, C.c_void_p, C.c_size_t]),
         "soup_write": (C.c_int, [H, C.c_int, C.c_void_p, C.c_size_t]),
@@ -137,7 +140,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_nonfinite": (C.c_int, [H, C.POINTER(C.c_uint32)]),
         "soup_check_lists": (C.c_int, [H]),
         "soup_list_stats": (C.c_int, [H, C.POINTER(SoupListStats)]),
-        "soup_relax": (C.c_int, [H, C.c_uint32, C.c_float]),
+        "soup_relax": (C.c_int, [H, C.c_uint32, C.c_double]),
         "soup_potential": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(C.c_double)]),
         "soup_potential_at": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),

@@ -7,8 +7,11 @@ Henderson-Hasselbalch charges), packs the flat ``SoupConfig`` and hands everythi
 library, which owns the device state and runs the priming pass. ``SoupSystem`` mirrors the
 reference's method names (soup/src/soup-types.ts:181-442) for the methods on this path.
 
-Out of scope here (they raise): the clay platelet, parcel confinement, dry-wet cycling and
-evaporation, the area move.
+Out of scope here (they raise): the clay platelet and parcel confinement.
+
+Capacity and live count (soup/src/sim.ts:139-151): the device buffers hold the creation census
+(``capacity``); the live count ``N`` is smaller on a system resumed mid dry phase or after solvent
+evaporation, and grows back on rehydration (protocell_soup.evaporate).
 """
 from __future__ import annotations
 
@@ -24,6 +27,7 @@ from .electrostatics import EsBasis, initialCharges, makeEsBasis
 from .init_state import NONE_U32, lattice_state
 from .native import FIELD, SoupError
 from .box_scale import applyBoxScale, cyclePhaseAt, deriveCycleConfig, growBoxTo, stepCycled
+from .evaporate import planEvaporation
 from .plan import deriveGridGeometry, deriveListCapacity, densestDensityOf, planSoupGrid, verletListBytes
 from .rules import (
     acceptanceProbability,
@@ -64,7 +68,7 @@ class Resolved:
 
 
 def resolve(opts: Dict[str, Any]) -> Resolved:
-    """soup/src/sim.ts:96-322 restricted to unconfined, clay-free systems without solvent evaporation."""
+    """soup/src/sim.ts:96-322 restricted to unconfined, clay-free systems."""
     soup = loadSoup()
     if len(soup["monomers"]) > 8:
         raise ValueError(f"data/soup.json: {len(soup['monomers'])} monomer kinds, but the shader holds at most 8")
@@ -86,8 +90,7 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     if opts.get("confine"):
         raise ValueError("createSoup: parcel confinement is not part of this build")
     cycling = bool(opts.get("dryWetCycle", soup["dryWetCycle"]["enabled"]))
-    if cycling and opts.get("evaporateSolvent", soup["dryWetCycle"].get("evaporateSolvent", False)):
-        raise ValueError("createSoup: solvent evaporation during the dry phase is not part of this build")
+    evap_requested = cycling and bool(opts.get("evaporateSolvent", soup["dryWetCycle"].get("evaporateSolvent", False)))
     counts_by_kind = [int(start.get(m["id"], 0)) for m in soup["monomers"]]
     capacity_n = sum(counts_by_kind)
     resume = opts.get("resume")
@@ -97,8 +100,6 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
         raise ValueError("createSoup: the starting composition is empty")
     if N > capacity_n:
         raise ValueError(f"createSoup: the live composition ({N}) is larger than the starting one ({capacity_n}) -- buffers are allocated for the starting one")
-    if N != capacity_n:
-        raise ValueError("createSoup: a live composition below the creation census (evaporation) is not part of this build")
     geo = deriveGridGeometry(soup, p, opts["kT"])
     box = tuple(float(x) for x in opts["box"])
     initial_step = int(resume["globalStep"]) if resume else 0
@@ -106,7 +107,9 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     plan = planSoupGrid(live_box, start)
     if not plan.valid:
         raise ValueError(plan.reason)
-    cycle_cfg, dry_box = deriveCycleConfig(soup, opts, box, capacity_n, start)
+    # the dry box of an evaporating cycle holds what is left (planEvaporation), so it is resolved first
+    evap = planEvaporation(soup, p, box, start) if evap_requested else None
+    cycle_cfg, dry_box = deriveCycleConfig(soup, opts, box, capacity_n, start, evap.dryBox if evap else None)
     min_box = min(*live_box, *dry_box)
     densest = densestDensityOf(soup, capacity_n, box, N, live_box, cycle_cfg is not None)
     cap = deriveListCapacity(soup, geo.listRange, capacity_n, densest)
@@ -151,6 +154,7 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     R = Resolved()
     R.soup, R.p, R.rules, R.catalyst_kind, R.event_rule_ids = soup, p, rules, catalyst_kind, event_rule_ids
     R.start, R.counts_by_kind, R.N, R.box, R.live_box, R.plan, R.geo = start, counts_by_kind, N, box, live_box, plan, geo
+    R.capacity_n, R.evap = capacity_n, evap
     R.list_cap, R.es, R.es_heads, R.sweep_every, R.seed = list_cap, es, es_heads, sweep_every, seed
     R.opts = dict(opts)
     R.init, R.charges0, R.prng, R.events0, R.initial_step = init, charges0, prng_state, events0, initial_step
@@ -168,6 +172,7 @@ def build_config(R: Resolved, opts: Dict[str, Any]) -> native.SoupConfig:
     soup, p, es = R.soup, R.p, R.es
     c = native.SoupConfig()
     c.n = R.N
+    c.capacity = R.capacity_n
     c.n_kinds = len(soup["monomers"])
     for a in range(3):
         c.box[a] = R.live_box[a]
@@ -302,6 +307,22 @@ class SoupSystem:
         """Grows the box to targetBox without moving any particle (soup-box-scale.ts:318-373)."""
         growBoxTo(self, targetBox, int(rampSteps), int(rampRelaxSteps))
 
+    def evaporateDEBUG(self, targetSolvent: int) -> None:
+        """Solvent removal to targetSolvent live solvent beads at the current box (sim.ts:569-572)."""
+        from .evaporate import evaporateSolventTo
+
+        if self._R.evap is None:
+            raise ValueError("evaporateDEBUG: the system was created without solvent evaporation (CreateSoupOpts.evaporateSolvent)")
+        evaporateSolventTo(self, self._R.evap, int(targetSolvent))
+
+    def rehydrateDEBUG(self, targetSolvent: int) -> Dict[str, Any]:
+        """Solvent insertion back to targetSolvent at the current box, with its report (sim.ts:573-576)."""
+        from .evaporate import rehydrateSolventTo
+
+        if self._R.evap is None:
+            raise ValueError("rehydrateDEBUG: the system was created without solvent evaporation (CreateSoupOpts.evaporateSolvent)")
+        return rehydrateSolventTo(self, self._R.evap, int(targetSolvent), self._R.live_box, self._R.seed)
+
     # -- readbacks (soup/src/soup-readback.ts) ----------------------------------------------------
     def particles(self) -> np.ndarray:
         return self._read("POS", np.float32)
@@ -399,7 +420,7 @@ class SoupSystem:
     # -- lists ----------------------------------------------------------------------------------
     def verletConfig(self) -> Dict[str, Any]:
         return dict(self._R.verlet, listRange=self._R.geo.listRange, walkRadius=self._R.geo.effectiveWalkRadius,
-                    listBytes=verletListBytes(self.N, self._R.list_cap))
+                    listBytes=verletListBytes(self._R.capacity_n, self._R.list_cap))
 
     def potentialBasis(self, centerLink: bool = True, charges: bool = True) -> native.SoupPotentialBasis:
         """makePotentialBasis for this system's species, parameters and DH basis."""
@@ -490,7 +511,7 @@ class SoupSystem:
             return float(np.abs(f3[fin]).max(initial=0.0)), int((~fin).sum())
 
         before = stats(self.forces())
-        self._call(self._lib.soup_relax, C.c_uint32(iterations), C.c_float(d0))
+        self._call(self._lib.soup_relax, C.c_uint32(iterations), C.c_double(d0))
         after = stats(self.forces())
         return {"iterations": iterations, "maxDisplacementStart": d0, "displacementBound": d0 * (iterations + 1) / 2,
                 "maxForceBefore": before[0], "maxForceAfter": after[0], "nonFiniteBefore": before[1], "nonFiniteAfter": after[1]}

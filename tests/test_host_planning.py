@@ -134,7 +134,7 @@ def test_config_struct_layout_matches_c(tmp_path):
     src = tmp_path / "off.c"
     fields = ["n", "box", "dims", "list_range", "kT", "b_tt", "wca_factor", "radius", "cls", "attr", "acid_soap",
               "es_enabled", "es_Frc", "n_heads", "ph_A", "ph_q", "as_eps", "sweep_every", "n_rules", "kindA", "roleB",
-              "catalyst_kind", "bond_interval", "reserved"]
+              "catalyst_kind", "bond_interval", "capacity", "reserved"]
     body = "\n".join(f'printf("{f} %zu\\n", offsetof(SoupConfig, {f}));' for f in fields)
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "soup_abi.h"\nint main(){' + body +
                    'printf("size %zu\\n", sizeof(SoupConfig)); printf("init %zu\\n", sizeof(SoupInitState)); return 0;}')
