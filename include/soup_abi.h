@@ -211,6 +211,21 @@ int soup_reset_particles(SoupHandle* h, uint32_t n, const SoupInitState* st);
 int soup_sample_insertion(const float* active, uint32_t n_active, const double box[3], uint32_t count, double min_sep,
                           uint32_t* rng, float* out, uint32_t* short_of_floor, double* min_achieved);
 
+/* forceCandidateStatsDEBUG (soup/src/soup-readback.ts:246-263): the neighbour grid of the current
+ * positions (in scratch memory: the state and its lists are not touched), then the cell walk of
+ * soup_force_stats_main (soup/wgsl/step.wgsl:307-343). out[0] = candidates examined (j != i in the
+ * +-walkRadius cells), out[1] = pairs within range (inside the pair's WCA cut, or inside the
+ * attraction shell of an attracting class pair). 64-bit counts, where the reference's counters are
+ * 32-bit atomics. */
+int soup_force_candidate_stats(SoupHandle* h, uint64_t out[2]);
+
+/* stepPhasesDEBUG (soup/src/soup-readback.ts:191-239): device milliseconds per iteration of five
+ * groups, each run n times on the live state: ms[0] a full step with the rebuild and the bond
+ * phase, ms[1] the rebuild (sort, DH list, Verlet list), ms[2] the list force, ms[3] the bond
+ * Monte Carlo, ms[4] the two integration halves. The step counter does not move; the call ends with
+ * a rebuild, a force pass and the guards, so the next step starts from consistent lists and F(x). */
+int soup_step_phases(SoupHandle* h, uint32_t n, double ms[5]);
+
 /* Replaces the interaction strengths of a live system (epsilon, the attraction table and acid-soap
  * depth, the DH coefficient, kappa and shift force, thermostat and bond-rule rates) with those of
  * *c; every geometric, range, radius and capacity field must equal the current one. The force the

@@ -145,6 +145,12 @@ class OracleState:
     def force_brute(self):
         lib().or_force_brute(self._c, self._s)
 
+    def force_candidate_stats(self):
+        """(candidates examined, pairs within range) of the cell walk at the current positions."""
+        out = (C.c_uint64 * 2)()
+        lib().or_force_candidate_stats(self._c, self._s, out)
+        return int(out[0]), int(out[1])
+
     def force_f64(self, use_list: bool = False) -> np.ndarray:
         out = np.zeros(3 * self.n, dtype=np.float64)
         lib().or_force_f64(self._c, self._s, C.c_int(1 if use_list else 0), out.ctypes.data_as(C.c_void_p))

@@ -1410,6 +1410,48 @@ void or_lattice(const double box[3], const uint32_t* counts, uint32_t n_kinds, u
 /* ------------------------------------------------------------------------------------------- */
 enum { OR_OK = 0, OR_NONFINITE = -3, OR_OVERFLOW = -4, OR_DRIFT = -5 };
 
+/* forceCandidateStatsDEBUG (soup/src/soup-readback.ts:246-263): the grid rebuilt at the current
+ * positions, then soup_force_stats_main (soup/wgsl/step.wgsl:307-343): every particle's +-walkRadius
+ * cell walk counts each candidate j != i, and each pair inside its WCA cut or inside the attraction
+ * shell [rcAttr, rcAttr + wc] of an attracting class pair. out[0] candidates, out[1] pairs. */
+void or_force_candidate_stats(const SoupConfig* c, OrState* s, uint64_t out[2]) {
+  or_grid_build(c, s);
+  const int R = (int)c->walk_radius;
+  const float rc_attr = wca_cut_f(c, c->b_tt);
+  uint64_t cand = 0, pairs = 0;
+  for (uint32_t i = 0; i < c->n; ++i) {
+    const float* xi = s->pos + 4u * i;
+    const float ti = xi[3];
+    int cc[3];
+    cell_coord(c, xi, cc);
+    for (int dz = -R; dz <= R; ++dz) {
+      const int cz = wrap_axis(cc[2] + dz, (int)c->dims[2]);
+      for (int dy = -R; dy <= R; ++dy) {
+        const int cy = wrap_axis(cc[1] + dy, (int)c->dims[1]);
+        for (int dx = -R; dx <= R; ++dx) {
+          const int cx = wrap_axis(cc[0] + dx, (int)c->dims[0]);
+          const uint32_t nc = (uint32_t)cx + c->dims[0] * ((uint32_t)cy + c->dims[1] * (uint32_t)cz);
+          for (uint32_t k = s->cell_start[nc]; k < s->cell_start[nc + 1]; ++k) {
+            const uint32_t j = s->cell_idx[k];
+            if (j == i) continue;
+            ++cand;
+            const float* xj = s->pos + 4u * j;
+            const float tj = xj[3];
+            const float r = len3_f(mi_f(xi[0] - xj[0], c->box[0]), mi_f(xi[1] - xj[1], c->box[1]), mi_f(xi[2] - xj[2], c->box[2]));
+            const float b = pair_b_f(c, ti, tj);
+            const int within_wca = r < wca_cut_f(c, b);
+            const int attract = c->attr[c->cls[(uint32_t)ti]][c->cls[(uint32_t)tj]] > 0.0f;
+            const int within_attr = attract && r >= rc_attr && r <= rc_attr + c->wc;
+            if (within_wca || within_attr) ++pairs;
+          }
+        }
+      }
+    }
+  }
+  out[0] = cand;
+  out[1] = pairs;
+}
+
 int or_check_lists(const SoupConfig* c, const OrState* s) {
   if (s->v_overflow) return OR_OVERFLOW;
   if (c->es_enabled && c->n_heads > 0 && s->es_overflow) return OR_OVERFLOW;

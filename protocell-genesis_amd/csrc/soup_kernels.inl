@@ -2059,6 +2059,62 @@ __global__ __launch_bounds__(256) void k_pa_cells(const float4* xs, uint32_t n, 
   cell_of[i] = c;
   atomicAdd(&counts[c], 1u);
 }
+// ================================================================================================
+// forceCandidateStatsDEBUG (soup/src/soup-readback.ts:246-263): the step grid of the current
+// positions in scratch arrays (the state's own cell order is left alone), then the cell walk of
+// soup_force_stats_main (soup/wgsl/step.wgsl:307-343) counting candidates and in-range pairs.
+// ================================================================================================
+__global__ __launch_bounds__(256) void k_stats_cells(SoupConfig c, DevBuf b, uint32_t* cell, uint32_t* counts) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t k = cell_of(c, b.pos[t]);
+  cell[t] = k;
+  atomicAdd(&counts[k], 1u);
+}
+
+// one thread per particle; a pair is in range inside its WCA cut, or inside the attraction shell
+// [rcAttr, rcAttr + wc] of a class pair whose table depth is positive (shouldAttract)
+__global__ __launch_bounds__(256) void k_force_stats(SoupConfig c, DevBuf b, const uint32_t* cell, const uint32_t* start,
+                                                     const uint32_t* order, unsigned long long* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cand = 0, pairs = 0;
+  if (t < c.n) {
+    const float4 xi = b.pos[t];
+    const uint32_t ci = cell[t];
+    const int d0 = (int)c.dims[0], d1 = (int)c.dims[1], d2 = (int)c.dims[2];
+    const int cx0 = (int)(ci % c.dims[0]), cy0 = (int)((ci / c.dims[0]) % c.dims[1]), cz0 = (int)(ci / (c.dims[0] * c.dims[1]));
+    const int R = (int)c.walk_radius;
+    const float rc = wca_cut(c, c.b_tt);
+    const uint32_t clsi = c.cls[(uint32_t)xi.w];
+    for (int dz = -R; dz <= R; ++dz) {
+      const int cz = ((cz0 + dz) % d2 + d2) % d2;
+      for (int dy = -R; dy <= R; ++dy) {
+        const int cy = ((cy0 + dy) % d1 + d1) % d1;
+        for (int dx = -R; dx <= R; ++dx) {
+          const int cx = ((cx0 + dx) % d0 + d0) % d0;
+          const uint32_t nc = (uint32_t)cx + c.dims[0] * ((uint32_t)cy + c.dims[1] * (uint32_t)cz);
+          for (uint32_t k = start[nc]; k < start[nc + 1]; ++k) {
+            const uint32_t j = order[k];
+            if (j == t) continue;
+            ++cand;
+            const float4 xj = b.pos[j];
+            const float r = dist_f(c, xi, xj);
+            const bool wca = r < wca_cut(c, pair_b(c, xi.w, xj.w));
+            const bool attr = c.attr[clsi][c.cls[(uint32_t)xj.w]] > 0.0f && r >= rc && r <= rc + c.wc;
+            pairs += (wca || attr) ? 1u : 0u;
+          }
+        }
+      }
+    }
+  }
+  cand = wave_sum_u(cand);
+  pairs = wave_sum_u(pairs);
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    if (cand) atomicAdd(&out[0], (unsigned long long)cand);
+    if (pairs) atomicAdd(&out[1], (unsigned long long)pairs);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_pa_fill(const uint32_t* cell_of, uint32_t n, uint32_t* cursor, uint32_t* order) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) order[atomicAdd(&cursor[cell_of[i]], 1u)] = i;

@@ -1191,6 +1191,24 @@ int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const doubl
   return 0;
 }
 
+// the scratch cell grid of soup_potential_at and soup_force_candidate_stats: per-particle arrays at
+// the capacity, count and start arrays grown to ncell + 1
+static int pa_scratch(SoupHandle* h, size_t ncell) {
+  if (!h->pa_xs) {
+    DA(h, h->pa_xs, h->cap);
+    DA(h, h->pa_cell, h->cap);
+    DA(h, h->pa_order, h->cap);
+  }
+  size_t cap_c = h->pa_ncell_cap;
+  int r = dgrow(h, &h->pa_count, ncell + 1, cap_c);
+  if (r) return r;
+  size_t cap_s = h->pa_ncell_cap;
+  r = dgrow(h, &h->pa_start, ncell + 1, cap_s);
+  if (r) return r;
+  h->pa_ncell_cap = std::min(cap_c, cap_s);
+  return 0;
+}
+
 int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3], double out[5]) {
   if (!h) return SOUP_E_INVALID;
   if (!pb || !positions || !box || !out) return fail(h, SOUP_E_INVALID, "soup_potential_at: null argument");
@@ -1216,18 +1234,8 @@ int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* 
   if (need > h->stage_bytes) return fail(h, SOUP_E_INVALID, "soup_potential_at: staging buffer too small");
   int r = sync(h);
   if (r) return r;
-  if (!h->pa_xs) {
-    DA(h, h->pa_xs, h->cap);
-    DA(h, h->pa_cell, h->cap);
-    DA(h, h->pa_order, h->cap);
-  }
-  size_t cap_c = h->pa_ncell_cap;
-  r = dgrow(h, &h->pa_count, ncell + 1, cap_c);
+  r = pa_scratch(h, ncell);
   if (r) return r;
-  size_t cap_s = h->pa_ncell_cap;
-  r = dgrow(h, &h->pa_start, ncell + 1, cap_s);
-  if (r) return r;
-  h->pa_ncell_cap = std::min(cap_c, cap_s);
   HC(h, hipMemcpyAsync(h->pa_xs, positions, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
   fill(h, h->pa_count, 0u, (uint32_t)ncell);
   hipLaunchKernelGGL(k_pa_cells, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_xs, n, g, h->pa_cell, h->pa_count);
@@ -1555,6 +1563,79 @@ int soup_relax(SoupHandle* h, uint32_t iterations, double d0) {
   ph_rebuild(h);
   ph_force(h);
   int r = sync(h);
+  if (r) return r;
+  r = assert_finite(h, false, 0);
+  if (r) return r;
+  return assert_lists(h);
+}
+
+int soup_force_candidate_stats(SoupHandle* h, uint64_t out[2]) {
+  if (!h || !out) return SOUP_E_INVALID;
+  int r = sync(h);
+  if (r) return r;
+  r = pa_scratch(h, h->ncells);
+  if (r) return r;
+  const uint32_t n = h->n;
+  fill(h, h->pa_count, 0u, h->ncells);
+  hipLaunchKernelGGL(k_stats_cells, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->pa_cell, h->pa_count);
+  scan(h, h->pa_count, h->ncells, h->pa_start);
+  HC(h, hipMemcpyAsync(h->pa_count, h->pa_start, (size_t)h->ncells * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
+  hipLaunchKernelGGL(k_pa_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_cell, n, h->pa_count, h->pa_order);
+  HC(h, hipMemsetAsync(h->lsum, 0, 2 * sizeof(unsigned long long), h->stream));
+  hipLaunchKernelGGL(k_force_stats, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->pa_cell, h->pa_start,
+                     h->pa_order, h->lsum);
+  unsigned long long v[2];
+  HC(h, hipMemcpyAsync(v, h->lsum, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  r = sync(h);
+  if (r) return r;
+  out[0] = v[0];
+  out[1] = v[1];
+  return 0;
+}
+
+int soup_step_phases(SoupHandle* h, uint32_t n, double ms[5]) {
+  if (!h || !ms || n == 0) return SOUP_E_INVALID;
+  int r = sync(h);
+  if (r) return r;
+  hipEvent_t a = nullptr, e = nullptr;
+  HC(h, hipEventCreate(&a));
+  if (hipEventCreate(&e) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(h, SOUP_E_DEVICE, "soup_step_phases: event creation failed");
+  }
+  for (int g = 0; g < 5 && r == 0; ++g) {
+    (void)hipEventRecord(a, h->stream);
+    for (uint32_t k = 0; k < n; ++k) {
+      switch (g) {
+        case 0:  // full: one step with the rebuild and the bond phase
+          ph_kick_drift(h);
+          ph_rebuild(h);
+          ph_force(h);
+          ph_drift(h);
+          ph_bonds(h);
+          ph_kick_thermo(h);
+          break;
+        case 1: ph_rebuild(h); break;
+        case 2: ph_force(h); break;
+        case 3: ph_bonds(h); break;
+        default:
+          ph_kick_drift(h);
+          ph_kick_thermo(h);
+          break;
+      }
+    }
+    (void)hipEventRecord(e, h->stream);
+    r = sync(h);
+    float t = 0.f;
+    if (r == 0 && hipEventElapsedTime(&t, a, e) == hipSuccess) ms[g] = (double)t / (double)n;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(e);
+  if (r) return r;
+  // the lists and F(x) of the state the next step starts from
+  ph_rebuild(h);
+  ph_force(h);
+  r = sync(h);
   if (r) return r;
   r = assert_finite(h, false, 0);
   if (r) return r;

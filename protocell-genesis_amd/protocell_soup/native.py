@@ -96,7 +96,7 @@ SET_BOX_FORCE = 1  # soup_set_box: run a force pass after the rebuild
 
 # every entry point include/soup_abi.h declares
 EXPORTS = (
-    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_reset_particles", "soup_sample_insertion", "soup_global_step", "soup_read", "soup_write",
+    "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_reset_particles", "soup_sample_insertion", "soup_force_candidate_stats", "soup_step_phases", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
     "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
@@ -126,6 +126,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "soup_step_record": (C.c_int, [H, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "soup_set_params": (C.c_int, [H, C.POINTER(SoupConfig)]),
         "soup_reset_particles": (C.c_int, [H, C.c_uint32, C.POINTER(SoupInitState)]),
+        "soup_force_candidate_stats": (C.c_int, [H, C.POINTER(C.c_uint64)]),
+        "soup_step_phases": (C.c_int, [H, C.c_uint32, C.POINTER(C.c_double)]),
         "soup_sample_insertion": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_uint32, C.c_double,
                                             C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]),
         "soup_global_step": (C.c_int, [H, C.POINTER(C.c_uint64)]),

@@ -542,6 +542,67 @@ class SoupSystem:
             mask |= native.PHASE[ph]
         self._call(self._lib.soup_run_phases, C.c_uint32(mask), C.c_uint32(0))
 
+    def stepPhasesDEBUG(self, n: int) -> Dict[str, float]:
+        """Device milliseconds per iteration of each phase group run n times (soup-readback.ts:191-239):
+        full (a step with the rebuild and the bond phase), gridBuild (the rebuild: sort, DH list,
+        Verlet list), force (the list force), bondAttempts, integration. The step counter does not
+        move; the lists and F(x) are rebuilt at the end."""
+        ms = (C.c_double * 5)()
+        self._call(self._lib.soup_step_phases, C.c_uint32(int(n)), ms)
+        return dict(zip(("full", "gridBuild", "force", "bondAttempts", "integration"), (float(v) for v in ms)))
+
+    def forceCandidateStatsDEBUG(self) -> Dict[str, Any]:
+        """The cell walk's candidate count and the pairs within range at the current positions
+        (soup-readback.ts:246-263); the state is not touched."""
+        out = (C.c_uint64 * 2)()
+        self._call(self._lib.soup_force_candidate_stats, out)
+        cand, pairs = int(out[0]), int(out[1])
+        return {"candidatesExamined": cand, "pairsWithinRange": pairs, "ratio": cand / max(1, pairs)}
+
+    def minimiseNowDEBUG(self, iterations: int) -> Dict[str, Any]:
+        """The cold-start minimiser applied mid-run at the current box (sim.ts:518-528), without
+        relaxColdStart's step-0 refusal."""
+        d0 = (self._R.soup.get("coldStartRelax") or {}).get("maxDisplacementSigma", 0.1) * self._R.p["sigma"]
+        n = self.N
+
+        def mx(f):
+            f3 = f.reshape(-1, 4)[:n, :3]
+            return float(np.abs(f3).max(initial=0.0))  # NaN propagates, as Math.max does
+
+        before = mx(self.forces())
+        self._call(self._lib.soup_relax, C.c_uint32(int(iterations)), C.c_double(d0))
+        after = mx(self.forces())
+        return {"iterations": int(iterations), "globalStep": self.steps, "maxForceBefore": before, "maxForceAfter": after,
+                "displacementBound": d0 * (int(iterations) + 1) / 2}
+
+    def runUntil(self, stage: str, opts: Dict[str, int], log=None) -> Dict[str, Any]:
+        """sim.ts:452-484: stepCycled in batches of sampleEvery, detectStage after each batch; stops at
+        the first batch whose stage is `stage`, else after maxSteps. `steps` counts this call's
+        batches; each trace entry also carries the cycle phase and index."""
+        from .analysis import AnalysisContext, detectStage
+
+        trace = []
+        steps, reached = 0, False
+        max_steps, every = int(opts["maxSteps"]), int(opts["sampleEvery"])
+        ctx = AnalysisContext()
+        try:
+            while steps < max_steps:
+                chunk = min(every, max_steps - steps)
+                self.stepCycled(chunk)
+                steps += chunk
+                current, evidence = detectStage(self, ctx)
+                entry = {"steps": steps, "stage": current, "evidence": evidence, "cyclePhase": self.cyclePhase,
+                         "cycleIndex": self.cycleIndex}
+                trace.append(entry)
+                if log is not None:
+                    log(entry)
+                if current == stage:
+                    reached = True
+                    break
+        finally:
+            ctx.close()
+        return {"reached": reached, "steps": steps, "trace": trace}
+
     def writeDEBUG(self, field: str, arr: np.ndarray) -> None:
         self._write(field, arr)
 

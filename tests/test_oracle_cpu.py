@@ -486,3 +486,59 @@ def test_nonfinite_guard_and_relax():
     f0 = np.abs(o.force).max()
     assert o.relax(200, 0.1) == 0
     assert np.abs(o.force).max() < 0.05 * f0
+
+
+# ---------------------------------------------------------------------------------------------
+# forceCandidateStatsDEBUG's cell walk
+# ---------------------------------------------------------------------------------------------
+def _brute_pairs_in_range(cfg, pos):
+    """Ordered pairs i != j inside the pair's WCA cut or inside the attraction shell of an attracting
+    class pair, over all pairs, in the same f32 operations (IEEE, no contraction)."""
+    f32 = np.float32
+    p = pos.reshape(-1, 4)
+    kinds = p[:, 3].astype(np.int64)
+    box = [f32(cfg.box[a]) for a in range(3)]
+    rad = np.array([cfg.radius[k] for k in range(8)], np.float32)
+    cls = np.array([cfg.cls[k] for k in range(8)], np.int64)
+    attr = np.array([[cfg.attr[i][j] for j in range(4)] for i in range(4)], np.float32)
+    rc = f32(cfg.wca_factor) * f32(cfg.b_tt)
+    rc_hi = rc + f32(cfg.wc)
+    total = 0
+    for i in range(p.shape[0]):
+        r2 = np.zeros(p.shape[0], np.float32)
+        for a in range(3):
+            d = p[i, a] - p[:, a]
+            m = d - np.rint(d / box[a]) * box[a]
+            r2 = r2 + m * m
+        r = np.sqrt(r2)
+        b = (f32(cfg.sigma) * (rad[kinds[i]] + rad[kinds])) * f32(0.5)
+        wca = r < f32(cfg.wca_factor) * b
+        att = (attr[cls[kinds[i]], cls[kinds]] > 0) & (r >= rc) & (r <= rc_hi)
+        hit = wca | att
+        hit[i] = False
+        total += int(hit.sum())
+    return total
+
+
+def test_force_candidate_stats_against_brute_force():
+    # a 5-cell box with walk radius 2 walks every cell: each particle examines all N - 1 others
+    R, o = _state(dict(BROTH, start={"C": 600, "O": 100, "H": 500, "M": 20, "W": 1000}))
+    assert list(R.cfg.dims) == [5, 5, 5] and R.cfg.walk_radius == 2
+    cand, pairs = o.force_candidate_stats()
+    assert cand == R.N * (R.N - 1)
+    assert pairs == _brute_pairs_in_range(R.cfg, o.pos)
+    # a larger box walks 5^3 of 7^3 cells: the candidates are the particles of those cells
+    R, o = _state({"box": [21, 21, 21], "seed": 3, "kT": 1.0, "clay": False, "start": {"C": 900, "O": 300, "H": 300, "M": 20, "W": 0}})
+    assert list(R.cfg.dims) == [7, 7, 7]
+    cand, pairs = o.force_candidate_stats()
+    cell = o.cell_of.astype(np.int64)
+    cx, cy, cz = cell % 7, (cell // 7) % 7, cell // 49
+    exp = 0
+    for i in range(R.N):
+        dx = np.minimum((cx - cx[i]) % 7, (cx[i] - cx) % 7)
+        dy = np.minimum((cy - cy[i]) % 7, (cy[i] - cy) % 7)
+        dz = np.minimum((cz - cz[i]) % 7, (cz[i] - cz) % 7)
+        exp += int(((dx <= 2) & (dy <= 2) & (dz <= 2)).sum()) - 1
+    assert cand == exp
+    # every pair in range lies within the walk (the cells are wider than any cut), so the brute count holds
+    assert pairs == _brute_pairs_in_range(R.cfg, o.pos)
