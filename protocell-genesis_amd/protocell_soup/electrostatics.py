@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import Any, Dict, Optional
 
 import numpy as np
 
@@ -156,6 +156,56 @@ def pcgNext(s: PcgState) -> float:
 def headIndices(positions: np.ndarray, chargedKind: int) -> np.ndarray:
     kinds = np.trunc(positions.reshape(-1, 4)[:, 3]).astype(np.int64)
     return np.nonzero(kinds == chargedKind)[0].astype(np.int32)
+
+
+def _js_round(x: np.ndarray) -> np.ndarray:
+    r = np.floor(x)
+    return r + (x - r >= 0.5)
+
+
+def pairingStats(positions: np.ndarray, charges: np.ndarray, box, b: EsBasis, contactRadius: float) -> Dict[str, Any]:
+    """electrostatics.ts:520-565: head-head contacts inside contactRadius (FP64 minimum image of the
+    f32 coordinates), split into unlike (one charged, one neutral) and like pairs seen from each
+    head, and the heads whose nearest head neighbour in contact has the unlike charge (paired).
+    The reference walks a head cell list of side >= contactRadius; the candidate pairs here come
+    from a periodic k-d tree queried slightly wider and are then filtered by the same distance."""
+    from scipy.spatial import cKDTree
+
+    pos = np.asarray(positions, dtype=np.float32).reshape(-1, 4)
+    q = np.asarray(charges, dtype=np.float32).reshape(-1)
+    heads = headIndices(pos.reshape(-1), b.chargedKind).astype(np.int64)
+    nh = heads.size
+    charged_h = q[heads] != 0
+    alpha = float(charged_h.sum()) / nh if nh > 0 else 0.0
+    unlike = like = with_nb = paired = 0
+    if nh > 1:
+        L = np.asarray(box, np.float64)
+        x = pos[heads, :3].astype(np.float64)
+        tree = cKDTree(np.mod(x, L) % L, boxsize=L)
+        pr = tree.query_pairs(r=contactRadius * (1 + 1e-9) + 1e-9, output_type="ndarray")
+        if pr.size:
+            a, c = pr[:, 0], pr[:, 1]
+            d = x[a] - x[c]
+            d = d - _js_round(d / L) * L
+            r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+            keep = r <= contactRadius
+            a, c, r = a[keep], c[keep], r[keep]
+            # each unordered contact is seen from both of its heads
+            src = np.concatenate([a, c])
+            dst = np.concatenate([c, a])
+            rr = np.concatenate([r, r])
+            diff = charged_h[src] != charged_h[dst]
+            unlike, like = int(diff.sum()), int((~diff).sum())
+            with_nb = int(np.unique(src).size)
+            order = np.lexsort((rr, src))
+            first = np.ones(order.size, bool)
+            first[1:] = src[order][1:] != src[order][:-1]
+            paired = int(diff[order][first].sum())
+    contacts = unlike + like
+    return {"heads": nh, "withNeighbour": with_nb, "unlikeContacts": unlike, "likeContacts": like,
+            "unlikeFraction": unlike / contacts if contacts > 0 else 0, "unlikeFractionRandom": 2 * alpha * (1 - alpha),
+            "paired": paired, "pairedFraction": paired / nh if nh > 0 else 0, "contactRadius": contactRadius,
+            "alpha": alpha}
 
 
 def initialCharges(positions: np.ndarray, es: EsBasis, seed: int):
