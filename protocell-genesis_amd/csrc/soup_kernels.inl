@@ -387,7 +387,9 @@ __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t c
 // hits in order: the near ones (list_split) from the front of the row, the far ones from its back.
 // Needs (2R + 1)^2 <= 64 rows.
 // ================================================================================================
-#define VL_BATCH 4u
+#ifndef VL_BATCH
+#define VL_BATCH 4u  // Verlet-build candidate chunks of 64 whose loads are issued together
+#endif
 __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
   __shared__ uint32_t s_end[256 / WAVE][WAVE];  // inclusive prefix of kept row lengths
   __shared__ uint4 s_seg[256 / WAVE][WAVE];     // first slot of range a, its length, first slot of range b
@@ -529,7 +531,9 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
 // Candidate positions come from the compact head array (L2-resident), ES_BATCH chunks of 64 per
 // round of loads; entries are head indices.
 // ================================================================================================
-#define ES_BATCH 8u
+#ifndef ES_BATCH
+#define ES_BATCH 8u  // DH-list candidate chunks of 64 whose loads are issued together
+#endif
 __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t h = uni((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);

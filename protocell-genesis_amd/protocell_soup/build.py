@@ -22,17 +22,34 @@ HIPCC_FLAGS = [
 ]
 
 
-def build_library(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
+# launch shapes chosen by protocell_soup.autotune for the campaign workload (macro -> value)
+TUNING = os.path.join(PKG_ROOT, "protocell_soup", "tuning.json")
+
+
+def tuning_defines(path: str = TUNING) -> dict:
+    """The tuned launch-shape macros, {} when no tuning file is present."""
+    if not os.path.exists(path):
+        return {}
+    import json
+
+    with open(path, encoding="utf-8") as fh:
+        return {k: int(v) for k, v in json.load(fh).get("defines", {}).items()}
+
+
+def build_library(verbose: bool = False, out: str = LIB, defines: dict = None) -> str:
+    """hipcc of csrc/soup_lib.hip to `out`, with the tuned launch shapes (or `defines` instead)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     if not os.path.exists(hipcc):
         hipcc = "hipcc"
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(PKG_ROOT, "..", "include"), "-o", tmp, os.path.join(CSRC, "soup_lib.hip")]
+    tmp = out + ".tmp"
+    d = tuning_defines() if defines is None else defines
+    cmd = [hipcc, *HIPCC_FLAGS, *[f"-D{k}={v}u" for k, v in sorted(d.items())], "-I", os.path.join(PKG_ROOT, "..", "include"),
+           "-o", tmp, os.path.join(CSRC, "soup_lib.hip")]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stderr[-6000:])
     if verbose and res.stderr:
         print(res.stderr[-2000:])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out

@@ -108,11 +108,13 @@ PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_fa
 _lib: Optional[C.CDLL] = None
 
 
-def load_library(path: str = LIB_PATH) -> C.CDLL:
-    """Loads libsoup_hip.so and declares the signatures. Raises if it is not built."""
+def load_library(path: Optional[str] = None) -> C.CDLL:
+    """Loads libsoup_hip.so (SOUP_LIB_PATH names another build, such as an autotuner variant) and
+    declares the signatures. Raises if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("SOUP_LIB_PATH") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(f"libsoup_hip.so is not built at {path}: run __graft_entry__.build() first")
     lib = C.CDLL(path)
