@@ -2,7 +2,8 @@
 Debye-Hueckel) after the cold-start relax and 37 steps, against the oracle on the same state:
 the cell assignment, every Verlet count and row (set and order, 2.2 GB of list), the head lists of
 the long-range term, the list force within 1e-5 of the FP64 recomputation over the oracle's own
-lists, and the decisions of one constant-pH sweep over all 18,659 heads."""
+lists, the decisions of one constant-pH sweep over all 18,659 heads, and one bond Monte Carlo
+phase (claims, events, desorption and RNG streams)."""
 import numpy as np
 import pytest
 
@@ -47,5 +48,16 @@ def test_campaign_scale_parity():
         assert np.array_equal(s.charges(), o.charge)
         assert s.protonationRngState() == o.s.prng
         assert (r_gpu["accepted"], r_gpu["deprotonated"]) == (r_ref["accepted"], r_ref["deprotonated"])
+        # one bond Monte Carlo phase on this state: every claim, event and RNG draw
+        s.runPhasesDEBUG("BONDS")
+        o.bond_form()
+        o.bond_break()
+        assert np.array_equal(s.bondSlots(), o.bond_slots)
+        assert np.array_equal(s.centerLinks(), o.center_link)
+        assert np.array_equal(s.centerHeldSteps(), o.center_held)
+        assert np.array_equal(s.rngState()["bond"], o.bond_rng)
+        assert np.array_equal(s.readDEBUG("EVENTS"), o.events[: 2 * R.cfg.n_rules])
+        assert np.array_equal(s.readDEBUG("DESORB"), o.desorb)
+        print("bond phase equal, events", s.events(), flush=True)
     finally:
         s.dispose()
