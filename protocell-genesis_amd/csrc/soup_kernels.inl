@@ -1047,6 +1047,17 @@ __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
   if (t >= c.n) return;
   const float4 xi = b.pos[t];
   const float ti = xi.w;
+  // a kind that no rule names (the solvent) finds no candidate and draws nothing: the walk only
+  // decides its catalyst, which no proposal of it will read
+  bool bondable = false;
+  for (uint32_t r = 0; r < SOUP_BOND_RULES; ++r) bondable = bondable || ti == c.kindA[r] || ti == c.kindB[r];
+  if (!bondable) {
+    if (lane == 0) {
+      b.nacc[t] = 0u;
+      b.cat_id[t] = SOUP_NONE;
+    }
+    return;
+  }
   const uint32_t i = b.id[t];
   const uint4 own = b.bs[t];
   const float rcat = wca_cut(c, pair_b(c, ti, c.catalyst_kind));
