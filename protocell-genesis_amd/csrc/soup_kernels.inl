@@ -159,51 +159,26 @@ __global__ __launch_bounds__(256) void k_gq_refresh(SoupConfig c, DevBuf b) {
 // Also soup_max_drift_main (verlet.wgsl:115-123) on the new positions, which are the ones the force
 // of this step sees: one atomicMax per workgroup. On a rebuild step the rebuild clears the maximum
 // afterwards, as the reference's rebuild does before its own drift pass.
-__global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
-  __shared__ uint32_t s_max[256 / WAVE];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t bits = 0;
-  if (t < c.n) {
-    float4 x = b.pos[t];
-    if (b.frozen[t] != 0u) {
-      b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-      float4 v = b.vel[t], f = b.force[t];
-      float hdt = 0.5f * c.dt;
-      v.x = v.x + hdt * f.x;
-      v.y = v.y + hdt * f.y;
-      v.z = v.z + hdt * f.z;
-      b.vel[t] = make_float4(v.x, v.y, v.z, 0.f);
-      float px = x.x + c.dt * v.x, py = x.y + c.dt * v.y, pz = x.z + c.dt * v.z;
-      px = px - floorf(px / c.box[0]) * c.box[0];
-      py = py - floorf(py / c.box[1]) * c.box[1];
-      pz = pz - floorf(pz / c.box[2]) * c.box[2];
-      x = make_float4(px, py, pz, x.w);
-      b.pos[t] = x;
-    }
-    b.gq[t] = gq_pack(x, b.charge[t]);
-    bits = drift_sq_bits(c, x, b.pos_at_rebuild[t]);
-  }
-  bits = wave_max_u(bits);
-  if ((threadIdx.x & (WAVE - 1)) == 0) s_max[threadIdx.x / WAVE] = bits;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t m = 0;
-    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = max(m, s_max[w]);
-    // the maximum only grows between rebuilds: most workgroups see a larger value and skip the atomic
-    if (m > __hip_atomic_load(&b.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&b.flags[2], m);
-  }
+// the first half-kick, the drift and the periodic wrap of one mobile particle (kick_drift_wrap_main)
+__device__ __forceinline__ float4 kick_drift_one(const SoupConfig& c, DevBuf& b, uint32_t t, float4 x, float4 v, float4 f) {
+  const float hdt = 0.5f * c.dt;
+  v.x = v.x + hdt * f.x;
+  v.y = v.y + hdt * f.y;
+  v.z = v.z + hdt * f.z;
+  b.vel[t] = make_float4(v.x, v.y, v.z, 0.f);
+  float px = x.x + c.dt * v.x, py = x.y + c.dt * v.y, pz = x.z + c.dt * v.z;
+  px = px - floorf(px / c.box[0]) * c.box[0];
+  py = py - floorf(py / c.box[1]) * c.box[1];
+  pz = pz - floorf(pz / c.box[2]) * c.box[2];
+  x = make_float4(px, py, pz, x.w);
+  b.pos[t] = x;
+  return x;
 }
 
-__global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= c.n) return;
-  if (b.frozen[t] != 0u) {
-    b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  float4 v = b.vel[t], f = b.force[t];
-  float hdt = 0.5f * c.dt;
+// the second half-kick and the Langevin thermostat of one mobile particle (kick_thermostat_main):
+// the new velocity, with the particle's thermostat stream advanced by six draws
+__device__ __forceinline__ float4 kick_thermo_one(const SoupConfig& c, DevBuf& b, uint32_t t, float4 v, float4 f) {
+  const float hdt = 0.5f * c.dt;
   uint32_t s = b.trng[t];
   float noise[3];
 #pragma unroll
@@ -224,7 +199,66 @@ __global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
   vx = vx - gdt * vx + sigma_v * noise[0];
   vy = vy - gdt * vy + sigma_v * noise[1];
   vz = vz - gdt * vz + sigma_v * noise[2];
-  b.vel[t] = make_float4(vx, vy, vz, 0.f);
+  return make_float4(vx, vy, vz, 0.f);
+}
+
+// one atomicMax per workgroup of the drift maximum; the maximum only grows between rebuilds, so
+// most workgroups see a larger value and skip the atomic
+__device__ __forceinline__ void drift_max_block(DevBuf& b, uint32_t bits, uint32_t* s_max) {
+  bits = wave_max_u(bits);
+  if ((threadIdx.x & (WAVE - 1)) == 0) s_max[threadIdx.x / WAVE] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (uint32_t w = 0; w < blockDim.x / WAVE; ++w) m = max(m, s_max[w]);
+    if (m > __hip_atomic_load(&b.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&b.flags[2], m);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
+  __shared__ uint32_t s_max[256 / WAVE];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bits = 0;
+  if (t < c.n) {
+    float4 x = b.pos[t];
+    if (b.frozen[t] != 0u) b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    else x = kick_drift_one(c, b, t, x, b.vel[t], b.force[t]);
+    b.gq[t] = gq_pack(x, b.charge[t]);
+    bits = drift_sq_bits(c, x, b.pos_at_rebuild[t]);
+  }
+  drift_max_block(b, bits, s_max);
+}
+
+__global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  if (b.frozen[t] != 0u) {
+    b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  b.vel[t] = kick_thermo_one(c, b, t, b.vel[t], b.force[t]);
+}
+
+// kick_thermostat_main of the step that ended fused with kick_drift_wrap_main of the next, for two
+// steps that follow each other with nothing in between (inside a replayed step period): both read
+// the same force, and the velocity passes between them in registers, so the result is the two
+// kernels' bit for bit
+__global__ __launch_bounds__(256) void k_thermo_drift(SoupConfig c, DevBuf b) {
+  __shared__ uint32_t s_max[256 / WAVE];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bits = 0;
+  if (t < c.n) {
+    float4 x = b.pos[t];
+    if (b.frozen[t] != 0u) {
+      b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      const float4 f = b.force[t];
+      x = kick_drift_one(c, b, t, x, kick_thermo_one(c, b, t, b.vel[t], f), f);
+    }
+    b.gq[t] = gq_pack(x, b.charge[t]);
+    bits = drift_sq_bits(c, x, b.pos_at_rebuild[t]);
+  }
+  drift_max_block(b, bits, s_max);
 }
 
 // soup_max_drift_main (verlet.wgsl:115-123): wave max, one atomic per wave.

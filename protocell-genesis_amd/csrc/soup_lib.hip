@@ -198,6 +198,12 @@ static void ph_kick_drift(SoupHandle* h) {
   hipLaunchKernelGGL(k_kick_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   prof_end(h, SOUP_PROF_KICK_DRIFT, e);
 }
+// the thermostat half of the previous step fused with this step's first half (see k_thermo_drift)
+static void ph_thermo_drift(SoupHandle* h) {
+  hipEvent_t e = prof_begin(h, SOUP_PROF_KICK_DRIFT);
+  hipLaunchKernelGGL(k_thermo_drift, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  prof_end(h, SOUP_PROF_KICK_DRIFT, e);
+}
 static void ph_kick_thermo(SoupHandle* h) {
   hipEvent_t e = prof_begin(h, SOUP_PROF_KICK_THERMO);
   hipLaunchKernelGGL(k_kick_thermo, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -283,14 +289,18 @@ static void ph_bonds(SoupHandle* h) {
   prof_end(h, SOUP_PROF_BONDS, e);
 }
 
-static void one_step(SoupHandle* h) {
+// thermo_in: the previous step left its thermostat half to this step's first kernel; thermo_out:
+// this step leaves its own to the next (steps that follow each other inside a replayed period)
+static void one_step(SoupHandle* h, bool thermo_in = false, bool thermo_out = false) {
   const bool do_bonds = (h->global_step % h->c.bond_interval) == 0;
   const bool do_rebuild = (h->global_step % h->c.rebuild_every) == 0;
-  ph_kick_drift(h);  // also the drift max of this step's positions (soup-integrate.ts:96-98)
+  // also the drift max of this step's positions (soup-integrate.ts:96-98)
+  if (thermo_in) ph_thermo_drift(h);
+  else ph_kick_drift(h);
   if (do_rebuild) ph_rebuild(h);
   ph_force(h);
   if (do_bonds) ph_bonds(h);
-  ph_kick_thermo(h);
+  if (!thermo_out) ph_kick_thermo(h);
   h->global_step++;
 }
 
@@ -330,7 +340,7 @@ static bool run_period(SoupHandle* h, uint32_t P) {
   const uint64_t step0 = h->global_step;
   const DevBuf b0 = h->b, a0 = h->alt;
   if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
-  for (uint32_t k = 0; k < P; ++k) one_step(h);
+  for (uint32_t k = 0; k < P; ++k) one_step(h, k > 0, k + 1 < P);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
   hipGraphExec_t exec = nullptr;
