@@ -109,3 +109,32 @@ def test_step_replicas_rejects_a_system_listed_twice():
         assert s.steps == 10
     finally:
         s.dispose()
+
+
+def test_profiled_stepping_counts_every_launch():
+    # profiled steps run kernel by kernel (phase brackets are host-recorded events); switching profiling
+    # on, changing its mask and switching it off again leaves the state bit-identical to direct stepping
+    a, b = _pair(DENSE)
+    try:
+        for s in (a, b):
+            s.profile(True, phases=["force", "bonds"])
+        for n in (40, 7, 53, 120):
+            a.step(n)
+            b.step(n)
+        _assert_same(a, b, "profiled")
+        pa, pb = a.profileRead(), b.profileRead()
+        for ph in ("force", "bonds"):
+            assert pa[ph]["count"] == pb[ph]["count"] > 0 and pa[ph]["ms"] > 0
+        assert pa["force"]["count"] == 220
+        for s in (a, b):
+            s.profile(True, phases=["force"])
+            s.step(60)
+        _assert_same(a, b, "new mask")
+        assert a.profileRead()["force"]["count"] == 60 and a.profileRead()["bonds"]["count"] == 0
+        for s in (a, b):
+            s.profile(False)
+            s.step(40)
+        _assert_same(a, b, "profiling off")
+    finally:
+        a.dispose()
+        b.dispose()
