@@ -4,17 +4,19 @@ Workload (the campaign-scale configuration): a 76 sigma periodic box with 483,26
 {C: 55988, O: 18659, H: 55988, M: 1452, W: 351181} (seed 19, kT 1.1), charged heads at pH 7.0 and
 0.01 M (long-range Debye-Hueckel at rc = 15.2 sigma), bond Monte Carlo every 20 steps, Verlet
 rebuild every 10 steps, constant-pH sweep every 1000 steps, seeded jittered-lattice start followed by
-the cold-start minimiser. Inputs are resident in HBM before the timed region; the timed region is
-exactly --steps integration steps of SoupSystem.step (with the per-1000-step guards and the sweeps
-that fall inside it) and, every 1000 steps, the snapshot analysis of the campaign (amphiphile
-recognition, aggregate decomposition, encapsulated-water flood: detectStage).
+the cold-start minimiser. Inputs are resident in HBM before the timed region. The timed region is
+exactly --steps integration steps of SoupSystem.step as a caller runs them (step periods replayed
+from captured HIP graphs, with the per-1000-step guards and the sweeps that fall inside it) and,
+every 1000 steps, the campaign's snapshot analysis (amphiphile recognition, aggregate decomposition,
+encapsulated-water flood: detectStage). One constant-pH sweep and one analysis per 1000 steps belong
+to the workload: when the window holds none of either, one of each runs explicitly inside it, and
+`value` = 1 / (step time + (sweep + analysis time) / 1000), whatever --steps is.
 
 Prints ONE JSON line: steps/s, the roofline of the dominant kernel (k_force_group, the list force,
 eight lanes per particle; algorithmic bytes per launch = N * (60 + 4 * Verlet entries read) +
-heads * 28) with its average duration measured by HIP events on the library's stream over the timed
-region, and the CPU baseline: the oracle's OpenMP step on the host
-cores, timed on a bounded sample of the same state. A separate fully bracketed run of
---phase-steps steps gives the per-phase breakdown.
+heads * 28) with its average duration measured by HIP events on the library's stream in a separate,
+fully bracketed run of --phase-steps steps (which also gives the per-phase breakdown), and the CPU
+baseline: the oracle's OpenMP step on the host cores, timed on a bounded sample of the same state.
 
 The line's "batched" object is the replica sweep: --replicas amphiphile soups of uneven size stepped
 --replica-steps steps one replica after the other, then as many steps all together (stepReplicas),
@@ -101,7 +103,10 @@ def cpu_baseline(sys_, R, sample_steps):
     from oracle import analysis as ora  # checker / baseline only
     from oracle import oracle as orc  # checker / baseline only
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    # the host's CPU share: OMP_NUM_THREADS where the environment sets it, else every core this process
+    # may run on
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
     orc.set_threads(threads)
     init = {"positions": sys_.particles(), "velocities": sys_.velocities(), "bondSlots": sys_.bondSlots(),
             "centerLink": sys_.centerLinks(), "centerHeldSteps": sys_.centerHeldSteps(), "frozen": sys_.frozen(),
@@ -123,9 +128,31 @@ def cpu_baseline(sys_, R, sample_steps):
     t_an = time.perf_counter() - t0
     per_step = t_steps / sample_steps + t_sweep / R.sweep_every + t_an / ANALYSIS_EVERY
     return {"value": 1.0 / per_step, "unit": "steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": _cpu_model(),
             "sample": f"{sample_steps} oracle steps (incl. {sample_steps // 10} rebuilds, {sample_steps // 20} bond MC) "
                       f"+ 1 constant-pH sweep amortised over {R.sweep_every} steps + 1 snapshot analysis amortised over "
                       f"{ANALYSIS_EVERY} steps, same 483,268-bead state; {t_steps:.1f} s + {t_sweep:.1f} s + {t_an:.1f} s"}
+
+
+def _cpu_model():
+    """The host CPU's model name, as lscpu reports it."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def _oracle_snapshot_analysis(ora, sys_, R):
@@ -200,17 +227,20 @@ def main():
         done += k
     log(f"warmup {args.warmup} steps done ({time.perf_counter() - t0:.1f} s)")
 
-    # timed region: HIP events bracket only the list force kernel (the roofline's dominant kernel), on
-    # the library's own stream
+    # timed region: --steps steps of SoupSystem.step exactly as a caller runs them (step periods
+    # replayed from captured HIP graphs, no per-step events), with the campaign's snapshot analysis at
+    # every multiple of 1000 steps. Config 3 also runs one constant-pH sweep and one analysis per 1000
+    # steps: both are timed inside the region (the natural ones, or one of each run explicitly when
+    # the window holds none) and enter `value` at that cadence, whatever --steps is. Only the sweep is
+    # bracketed by HIP events, which keeps graph replay on.
     from protocell_soup.analysis import AnalysisContext, detectStage
 
     actx = AnalysisContext()
-    reads0 = s.readDEBUG("LIST_READS").astype(np.int64)
-    s.profile(True, phases=["force"])
+    s.profile(True, phases=["sweep"])
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     done = 0
-    t_analysis, n_analysis, stage = 0.0, 0, None
+    an_times, stage = [], None
     while done < args.steps:
         k = min(ANALYSIS_EVERY - s.steps % ANALYSIS_EVERY, args.steps - done)
         s.step(k)
@@ -218,23 +248,40 @@ def main():
         if s.steps % ANALYSIS_EVERY == 0:
             ta = time.perf_counter()
             stage, evidence = detectStage(s, actx)
-            t_analysis += time.perf_counter() - ta
-            n_analysis += 1
+            an_times.append(time.perf_counter() - ta)
         log(f"timed {done}/{args.steps}")
+    natural_sweeps = s.profileRead()["sweep"]["count"]
+    natural_analyses = len(an_times)
+    if natural_sweeps == 0:
+        s.protonationSweepDEBUG()
+    if natural_analyses == 0:
+        ta = time.perf_counter()
+        stage, evidence = detectStage(s, actx)
+        an_times.append(time.perf_counter() - ta)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    prof = s.profileRead()
-    steps_per_s = args.steps / elapsed
-    reads = s.readDEBUG("LIST_READS").astype(np.int64) - reads0  # near parts only, whole rows
+    sw = s.profileRead()["sweep"]
+    s.profile(False)
+    sweep_ms = sw["ms"] / max(1, sw["count"])
+    an_ms = 1000.0 * sum(an_times) / len(an_times)
+    t_steps = elapsed - sw["ms"] * 1e-3 - sum(an_times)  # the --steps steps alone
+    per_step = t_steps / args.steps + (sweep_ms + an_ms) * 1e-3 / ANALYSIS_EVERY
+    steps_per_s = 1.0 / per_step
+    log(f"timed region {elapsed:.2f} s: {args.steps} steps {t_steps:.3f} s, sweeps {sw['count']} x {sweep_ms:.1f} ms "
+        f"({natural_sweeps} in the step schedule), analyses {len(an_times)} x {an_ms:.1f} ms")
 
-    # phase breakdown from a separate, fully bracketed run (its events cost a little time per step)
+    # the list force's average launch, with HIP events on the library's stream, and the phase
+    # breakdown: a separate, fully bracketed run of --phase-steps steps (every phase bracketed turns
+    # graph replay off)
+    reads0 = s.readDEBUG("LIST_READS").astype(np.int64)
     s.profile(True)
     s.step(args.phase_steps)
     phases = s.profileRead()
     s.profile(False)
+    reads = s.readDEBUG("LIST_READS").astype(np.int64) - reads0  # near parts only, whole rows
     occ = s.listOccupancyDEBUG()
 
-    f = prof["force"]
+    f = phases["force"]
     force_ms = f["ms"] / max(1, f["count"])
     lbar = occ["verletMean"]
     lnear = float(s.readDEBUG("VERLET_NEAR").mean())
@@ -267,7 +314,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(s, R, args.cpu_sample)
-        log(f"cpu baseline {cpu['value']:.3f} steps/s on {cpu['cores']} threads")
+        log(f"cpu baseline {cpu['value']:.3f} steps/s on {cpu['cores']} threads (nproc {cpu['nproc']}, {cpu['cpu_model']})")
 
     actx.close()
     s.dispose()
@@ -297,10 +344,16 @@ def main():
             "algorithmic_bytes_per_step": round(bytes_step),
             "step_roofline_frac": round(bytes_step * steps_per_s / (HBM_PEAK_GBS * 1e9), 4),
             "phase_ms_per_step": {k: round(v["ms"] / args.phase_steps, 4) for k, v in phases.items() if v["count"]},
+            "steps_only_per_s": round(args.steps / t_steps, 3),
+            "sweep_every": R.sweep_every,
+            "sweep_ms": round(sweep_ms, 2),
+            "sweeps_timed": sw["count"],
             "analysis_every": ANALYSIS_EVERY,
-            "analysis_ms_per_sample": round(1000.0 * t_analysis / max(1, n_analysis), 2),
-            "analysis_samples": n_analysis,
+            "analysis_ms_per_sample": round(an_ms, 2),
+            "analysis_samples": len(an_times),
             "last_stage": stage,
+            "value_basis": "1 / (step time + (sweep + analysis) / 1000): the steps timed with graph replay, one "
+                           "constant-pH sweep and one snapshot analysis per 1000 steps, each timed in the region",
         },
         "roofline": {
             "kernel": "k_force_group",

@@ -417,7 +417,9 @@ int soup_relax(SoupHandle* h, uint32_t iterations, double d0);
 /* Device-time profile per phase, measured with HIP events on the handle's own stream around the
  * launches of each phase. soup_profile(h, mask) resets the totals and brackets the phases whose bit
  * (1 << SOUP_PROF_x) is set; mask -1 brackets every phase, 0 stops. soup_profile_read fills
- * SOUP_PROF_PHASES accumulated milliseconds and launch counts. */
+ * SOUP_PROF_PHASES accumulated milliseconds and launch counts. A mask of only the chunk-end phases
+ * (SOUP_PROF_PROTON, SOUP_PROF_SWEEP) keeps step replay from captured graphs on; any per-step phase
+ * turns it off while the profile runs. */
 enum {
   SOUP_PROF_KICK_DRIFT = 0,
   SOUP_PROF_SORT = 1,          /* cell sort + state permutation of a rebuild */
@@ -429,6 +431,7 @@ enum {
   SOUP_PROF_BONDS = 7,         /* bond MC, all kernels */
   SOUP_PROF_KICK_THERMO = 8,
   SOUP_PROF_PROTON = 9,        /* constant-pH chain kernel */
+  SOUP_PROF_SWEEP = 10,        /* a whole constant-pH sweep: DH list, pair tables, chain, rebuild and force */
   SOUP_PROF_PHASES = 16
 };
 int soup_profile(SoupHandle* h, int mask);

@@ -192,6 +192,11 @@ static void scan(SoupHandle* h, const uint32_t* in, uint32_t m, uint32_t* out) {
 
 static bool es_on(const SoupHandle* h) { return h->c.es_enabled && h->nh > 0; }
 
+// phases that run only at chunk ends, outside every captured step period
+static const uint32_t kChunkEndPhases = (1u << SOUP_PROF_PROTON) | (1u << SOUP_PROF_SWEEP);
+// whether steps may be replayed from captured graphs: not while a per-step phase is bracketed
+static bool replay_ok(const SoupHandle* h) { return h->use_graphs && (h->prof.mask & ~kChunkEndPhases) == 0; }
+
 // ---- phases ------------------------------------------------------------------------------------
 static void ph_kick_drift(SoupHandle* h) {
   hipEvent_t e = prof_begin(h, SOUP_PROF_KICK_DRIFT);
@@ -430,6 +435,7 @@ static int assert_lists(SoupHandle* h) {
 
 // ---- constant-pH sweep -------------------------------------------------------------------------
 static int sweep(SoupHandle* h, SoupSweepResult* out) {
+  hipEvent_t e_sweep = prof_begin(h, SOUP_PROF_SWEEP);
   ph_rebuild(h);  // DH list at the current positions
   hipLaunchKernelGGL(k_proton_pairs, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh,
                      h->ptab, h->ptab_g);
@@ -460,6 +466,7 @@ static int sweep(SoupHandle* h, SoupSweepResult* out) {
   // F(x) is stale with respect to the new charges (soup-protonation.ts:259-271)
   ph_rebuild(h);
   ph_force(h);
+  prof_end(h, SOUP_PROF_SWEEP, e_sweep);
   return sync(h);
 }
 
@@ -571,12 +578,43 @@ static int upload_state(SoupHandle* h, uint32_t n, const SoupInitState* st) {
   return 0;
 }
 
+// The device-limit guard of createSoup (soup/src/sim.ts:296-314): every buffer that scales with N is
+// checked against what this GPU can still hold before anything is allocated, and the message names
+// the byte count and the limit, so a configuration that cannot fit fails loudly at creation.
+static int check_device_limit(const SoupConfig& c, uint32_t cap, uint32_t nh) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;  // no figure to compare with: hipMalloc decides
+  const size_t list = (size_t)cap * c.list_cap * 4;
+  const size_t es_rows = (size_t)nh * c.es_cap;
+  struct Item {
+    const char* what;
+    size_t bytes;
+  } items[] = {
+      {"verletList (N*listCapacity*4)", list},
+      {"pos/vel/force (N*16)", (size_t)cap * 16},
+      {"esList (heads*longRangeListCapacity*4)", es_rows * 4},
+  };
+  // the list, its readback staging copy, the DH rows with their FP64 pair tables, and about 330 bytes
+  // of other state per particle
+  const size_t all = 2 * list + es_rows * 24 + (size_t)cap * 330;
+  for (const Item& it : items)
+    if (it.bytes > free_b || all > free_b)
+      return fail(nullptr, SOUP_E_NOMEM,
+                  fmt("buffer %s needs %zu bytes at N=%u (all device buffers: %zu bytes) -- device memory free=%zu of "
+                      "total=%zu bytes on this GPU; the allocation would be refused, and a run without its buffers "
+                      "would return zeros that look like success",
+                      it.what, it.bytes, cap, all, free_b, total_b));
+  return 0;
+}
+
 int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** out) {
   if (!out) return fail(nullptr, SOUP_E_INVALID, "soup_create: null output handle");
   *out = nullptr;
   int v = validate(cfg);
   if (v) return v;
   v = check_state(nullptr, *cfg, cfg->n, init, "soup_create");
+  if (v) return v;
+  v = check_device_limit(*cfg, std::max(cfg->n, cfg->capacity), (uint32_t)heads_of(*cfg, cfg->n, init->positions).size());
   if (v) return v;
   SoupHandle* h = new SoupHandle();
   h->c = *cfg;
@@ -803,7 +841,7 @@ int soup_step(SoupHandle* h, uint32_t nsteps) {
   while (done < nsteps) {
     uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
     uint64_t chunk_start = h->global_step;
-    const uint32_t P = h->use_graphs && h->prof.mask == 0 ? step_period(h->c) : 0u;
+    const uint32_t P = replay_ok(h) ? step_period(h->c) : 0u;
     for (uint32_t k = 0; k < chunk;) {
       if (P && h->global_step % P == 0 && chunk - k >= P && run_period(h, P)) {
         k += P;
@@ -894,7 +932,7 @@ int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t nsteps, uint
       SoupHandle* h = hs[s];
       start[s] = h->global_step;
       left[s] = chunk;
-      period[s] = h->use_graphs && h->prof.mask == 0 ? step_period(h->c) : 0u;
+      period[s] = replay_ok(h) ? step_period(h->c) : 0u;
     }
     // round robin: each system advances to its next period boundary or by one period per pass, so
     // the streams of all systems have work queued together

@@ -103,7 +103,7 @@ EXPORTS = (
     "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_use_graphs", "soup_version",
 )
 PROF_PHASES = ("kick_drift", "sort", "es_build", "verlet_build", "force", "es_far", "drift", "bonds", "kick_thermo",
-               "proton")
+               "proton", "sweep")
 
 _lib: Optional[C.CDLL] = None
 
