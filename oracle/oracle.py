@@ -259,6 +259,19 @@ def soup_potential(basis, positions, bond_slots, center_link, charges, box):
     return tuple(out)
 
 
+def bond_draws() -> np.ndarray:
+    """Per-rule bond Monte Carlo draw counters since the last reset: formation candidates reaching the
+    attempt draw, attempts passed, accepts passed; bonds reaching the break draw, break attempts
+    passed, break accepts passed."""
+    out = np.zeros(4 * 6, dtype=np.uint64)
+    lib().or_bond_draws(out.ctypes.data_as(C.c_void_p))
+    return out.reshape(4, 6)
+
+
+def bond_draws_reset() -> None:
+    lib().or_bond_draws_reset()
+
+
 def set_threads(t: int) -> None:
     lib().or_set_threads(C.c_int(t))
 

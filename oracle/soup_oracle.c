@@ -1050,6 +1050,15 @@ static void desorb_timeout(const SoupConfig* c, OrState* s, uint32_t cat) {
   }
 }
 
+/* Draw counters of the bond Monte Carlo, per rule: [0] formation candidates that reach the attempt
+ * draw, [1] those whose attempt draw passes, [2] those whose accept draw passes too, [3] bonds that
+ * reach the break attempt draw, [4] break attempts that pass, [5] break accept draws that pass. Test
+ * instrumentation of the Poisson thinning (soup/src/soup-buffers.ts:345-356): a pass fraction must
+ * equal the uploaded probability within binomial error. */
+static uint64_t g_bond_draws[SOUP_MAX_RULES][6];
+void or_bond_draws(uint64_t* out) { memcpy(out, g_bond_draws, sizeof(g_bond_draws)); }
+void or_bond_draws_reset(void) { memset(g_bond_draws, 0, sizeof(g_bond_draws)); }
+
 /* bond_form_list_main = bondFormWalkList (bond-dispatch.wgsl:212-242) + bondFormDecide (:89-142),
  * for i = 0..N-1 in ascending order. */
 void or_bond_form(const SoupConfig* c, OrState* s) {
@@ -1091,10 +1100,13 @@ void or_bond_form(const SoupConfig* c, OrState* s) {
       uint32_t j = candJ[ci], r = candRule[ci];
       float tj = kind_of(s, j);
       if (c->req_cat[r] > 0.5f && catalystNear == 0u) continue;
+      g_bond_draws[r][0]++;
       rng = pcg_soup(rng);
       if (uniform01(rng) >= c->att_form[r]) continue;
+      g_bond_draws[r][1]++;
       rng = pcg_soup(rng);
       if (uniform01(rng) >= c->acc_form[r]) continue;
+      g_bond_draws[r][2]++;
       uint32_t roleI = role_of(c, r, ti), roleJ = role_of(c, r, tj);
       int slotI = try_claim(c, s, i, roleI, j);
       if (slotI < 0) continue;
@@ -1128,10 +1140,13 @@ void or_bond_break(const SoupConfig* c, OrState* s) {
       if (p == NONE || p < i) continue;
       int r = match_rule(c, ti, kind_of(s, p));
       if (r < 0) continue;
+      g_bond_draws[r][3]++;
       rng = pcg_soup(rng);
       if (uniform01(rng) >= c->att_break[r]) continue;
+      g_bond_draws[r][4]++;
       rng = pcg_soup(rng);
       if (uniform01(rng) >= c->acc_break[r]) continue;
+      g_bond_draws[r][5]++;
       s->bond_slots[3u * i + k] = NONE;
       release_partner_slot(s, p, i);
       s->events[(uint32_t)r * 2u + 1u]++;
@@ -1361,8 +1376,11 @@ void or_lattice(const double box[3], const uint32_t* counts, uint32_t n_kinds, u
   uint32_t n = 0;
   for (uint32_t k = 0; k < n_kinds; ++k) n += counts[k];
   Mulberry m = {seed};
-  uint32_t nx = (uint32_t)ceil(cbrt((double)n));
-  if (nx < 1) nx = 1;
+  /* Math.max(1, Math.ceil(Math.cbrt(N))), then up while nx^3 < N (soup-init-state.ts:117-118): V8's
+   * cbrt is exact on perfect cubes and never crosses an integer elsewhere, so this is the smallest nx
+   * with nx^3 >= N. The C library's cbrt is not exact on perfect cubes (cbrt(27) = 3 + 1 ulp), so the
+   * smallest such nx is searched in integers instead. */
+  uint32_t nx = 1;
   while ((uint64_t)nx * nx * nx < n) nx++;
   double sp[3] = {box[0] / nx, box[1] / nx, box[2] / nx};
   uint32_t nsites = nx * nx * nx;

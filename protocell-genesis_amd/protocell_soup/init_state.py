@@ -62,7 +62,10 @@ def lattice_state(box: Sequence[float], counts_by_kind: Sequence[int], seed: int
     """Fresh-creation branch of buildInitialState (no clay, no parcel, no frozen catalysts)."""
     n = int(sum(counts_by_kind))
     rng = Mulberry32(seed)
-    nx = max(1, math.ceil(float(np.cbrt(float(n)))))
+    # Math.max(1, Math.ceil(Math.cbrt(N))) then up while nx^3 < N (soup-init-state.ts:117-118): V8's
+    # cbrt is exact on perfect cubes, so this is the smallest nx with nx^3 >= N, found in integers
+    # (a libm cbrt one ulp above an integer would give nx + 1 for a perfect cube)
+    nx = 1
     while nx * nx * nx < n:
         nx += 1
     spacing = np.array([box[0] / nx, box[1] / nx, box[2] / nx], dtype=np.float64)
@@ -128,7 +131,10 @@ def amphiphile_soup(box: Sequence[float], n_amph: int, tail_len: int, extra_coun
     n_free = int(sum(extra_counts.values()))
     n = n_amph * per + n_free
     rng = Mulberry32(seed)
-    nx = max(1, math.ceil(float(np.cbrt(float(n)))))
+    # Math.max(1, Math.ceil(Math.cbrt(N))) then up while nx^3 < N (soup-init-state.ts:117-118): V8's
+    # cbrt is exact on perfect cubes, so this is the smallest nx with nx^3 >= N, found in integers
+    # (a libm cbrt one ulp above an integer would give nx + 1 for a perfect cube)
+    nx = 1
     while nx * nx * nx < n:
         nx += 1
     seg_per_row = nx // per
