@@ -328,6 +328,20 @@ int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* basis, const do
 int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* basis, const float* positions, const double box[3],
                       double out[5]);
 
+/* soup_potential_at without any device allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight Monte Carlo loops, harnesses that
+ * forbid hipMalloc). The configuration's cell grid, its positions and the partial sums live in one
+ * fixed array per GPU (about 5.5 MiB, held until the process ends) instead of buffers the handle
+ * allocates on first use; the result is soup_potential_at's, bit for bit. Limits: at most
+ * SOUP_NOALLOC_MAX_N particles and SOUP_NOALLOC_MAX_CELLS grid cells (floor(box / cutoff) per axis),
+ * else SOUP_E_INVALID with a message. Every call on one GPU uses that GPU's one array, so calls that
+ * overlap on one GPU (from different threads, on any handles) race and must be serialised by the
+ * caller. */
+#define SOUP_NOALLOC_MAX_N 65536u
+#define SOUP_NOALLOC_MAX_CELLS 262144u
+int soup_potential_at_noalloc(SoupHandle* h, const SoupPotentialBasis* basis, const float* positions, const double box[3],
+                              double out[5]);
+
 /* The device half of a box change: adopts geometry->box, ->dims and ->walk_radius (the host's
  * planSoupGrid for the new box; every other field keeps its value), writes `positions` (particle
  * order, already mapped into the new box) unless it is NULL, reallocates the cell grid if it grew,

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,7 @@ struct SoupHandle {
   uint32_t cap = 0;              // particles every per-particle buffer holds (n <= cap)
   std::vector<uint32_t> heads;   // particle indices of the charged heads, ascending
   hipStream_t stream = nullptr;
+  int device = 0;                // the HIP device current at creation, which holds every buffer
   DevBuf b{};
   DevBuf alt{};
   double2* ptab = nullptr;
@@ -635,6 +637,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
     return code;
   };
   if ((r = [&]() -> int {
+        HC(h, hipGetDevice(&h->device));
         HC(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         DevBuf& b = h->b;
         DevBuf& a = h->alt;
@@ -1257,48 +1260,144 @@ static int pa_scratch(SoupHandle* h, size_t ncell) {
   return 0;
 }
 
-int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3], double out[5]) {
-  if (!h) return SOUP_E_INVALID;
-  if (!pb || !positions || !box || !out) return fail(h, SOUP_E_INVALID, "soup_potential_at: null argument");
-  if (!(pb->cutoff > 0.0)) return fail(h, SOUP_E_INVALID, "soup_potential_at: the pair cutoff must be > 0");
-  PaGrid g;
-  size_t ncell = 1;
+// device buffers of one soup_potential_at evaluation: positions, the configuration's cell grid, the
+// per-particle partial sums and the five totals
+struct PaBufs {
+  float4* xs;
+  uint32_t *cell, *order, *count, *start;
+  double4* part;
+  double* dout;
+};
+
+// The argument checks and the configuration's cell grid (soup-potential.ts:156-158) shared by both
+// forms of soup_potential_at; `who` names the entry point in messages.
+static int pa_grid(SoupHandle* h, const char* who, const SoupPotentialBasis* pb, const float* positions, const double box[3],
+                   const double* out, PaGrid& g, size_t& ncell) {
+  if (!pb || !positions || !box || !out) return fail(h, SOUP_E_INVALID, fmt("%s: null argument", who));
+  if (!(pb->cutoff > 0.0)) return fail(h, SOUP_E_INVALID, fmt("%s: the pair cutoff must be > 0", who));
+  ncell = 1;
   for (int a = 0; a < 3; ++a) {
-    if (!(box[a] > 0.0) || !std::isfinite(box[a])) return fail(h, SOUP_E_INVALID, "soup_potential_at: box edges must be finite and > 0");
-    const double d = std::floor(box[a] / pb->cutoff);  // soup-potential.ts:156-158
+    if (!(box[a] > 0.0) || !std::isfinite(box[a])) return fail(h, SOUP_E_INVALID, fmt("%s: box edges must be finite and > 0", who));
+    const double d = std::floor(box[a] / pb->cutoff);
     g.d[a] = d < 1.0 ? 1u : (uint32_t)std::min(d, 1024.0);
     g.L[a] = box[a];
     g.w[a] = box[a] / (double)g.d[a];
     ncell *= g.d[a];
   }
-  const uint32_t n = h->n;
-  for (uint32_t i = 0; i < n; ++i) {
+  for (uint32_t i = 0; i < h->n; ++i) {
     const float k = positions[4u * i + 3];
     if (!(k >= 0.0f) || k >= (float)h->c.n_kinds || !std::isfinite(positions[4u * i]) || !std::isfinite(positions[4u * i + 1]) ||
         !std::isfinite(positions[4u * i + 2]))
-      return fail(h, SOUP_E_INVALID, fmt("soup_potential_at: particle %u has a non-finite position or a kind outside the table", i));
+      return fail(h, SOUP_E_INVALID, fmt("%s: particle %u has a non-finite position or a kind outside the table", who, i));
   }
+  return 0;
+}
+
+// the evaluation itself, into buffers the caller provides (no device allocation here)
+static int pa_run(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const PaGrid& g, size_t ncell,
+                  const PaBufs& q, double out[5]) {
+  const uint32_t n = h->n;
+  HC(h, hipMemcpyAsync(q.xs, positions, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
+  fill(h, q.count, 0u, (uint32_t)ncell);
+  hipLaunchKernelGGL(k_pa_cells, dim3(blocks(n, 256)), dim3(256), 0, h->stream, q.xs, n, g, q.cell, q.count);
+  scan(h, q.count, (uint32_t)ncell, q.start);
+  HC(h, hipMemcpyAsync(q.count, q.start, ncell * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
+  hipLaunchKernelGGL(k_pa_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, q.cell, n, q.count, q.order);
+  hipLaunchKernelGGL(k_pa_sort, dim3(blocks(ncell, 256)), dim3(256), 0, h->stream, q.start, (uint32_t)ncell, q.order);
+  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
+  hipLaunchKernelGGL(k_potential_at, dim3(blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, q.xs, g,
+                     q.cell, q.start, q.order, q.part, es ? h->nh : 0u);
+  hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, q.part, n, q.dout);
+  HC(h, hipMemcpyAsync(out, q.dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return sync(h);
+}
+
+int soup_potential_at(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3], double out[5]) {
+  if (!h) return SOUP_E_INVALID;
+  PaGrid g;
+  size_t ncell = 0;
+  int r = pa_grid(h, "soup_potential_at", pb, positions, box, out, g, ncell);
+  if (r) return r;
+  const uint32_t n = h->n;
   const size_t need = (size_t)n * sizeof(double4) + 64;
   if (need > h->stage_bytes) return fail(h, SOUP_E_INVALID, "soup_potential_at: staging buffer too small");
-  int r = sync(h);
+  r = sync(h);
   if (r) return r;
   r = pa_scratch(h, ncell);
   if (r) return r;
-  HC(h, hipMemcpyAsync(h->pa_xs, positions, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, h->stream));
-  fill(h, h->pa_count, 0u, (uint32_t)ncell);
-  hipLaunchKernelGGL(k_pa_cells, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_xs, n, g, h->pa_cell, h->pa_count);
-  scan(h, h->pa_count, (uint32_t)ncell, h->pa_start);
-  HC(h, hipMemcpyAsync(h->pa_count, h->pa_start, ncell * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
-  hipLaunchKernelGGL(k_pa_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->pa_cell, n, h->pa_count, h->pa_order);
-  hipLaunchKernelGGL(k_pa_sort, dim3(blocks(ncell, 256)), dim3(256), 0, h->stream, h->pa_start, (uint32_t)ncell, h->pa_order);
-  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
-  double4* part = reinterpret_cast<double4*>(h->stage);
-  double* dout = reinterpret_cast<double*>(h->stage + (size_t)n * sizeof(double4));
-  hipLaunchKernelGGL(k_potential_at, dim3(blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, h->pa_xs, g,
-                     h->pa_cell, h->pa_start, h->pa_order, part, es ? h->nh : 0u);
-  hipLaunchKernelGGL(k_potential_sum, dim3(1), dim3(1024), 0, h->stream, part, n, dout);
-  HC(h, hipMemcpyAsync(out, dout, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  return sync(h);
+  PaBufs q{h->pa_xs, h->pa_cell, h->pa_order, h->pa_count, h->pa_start, reinterpret_cast<double4*>(h->stage),
+           reinterpret_cast<double*>(h->stage + (size_t)n * sizeof(double4))};
+  return pa_run(h, pb, positions, g, ncell, q, out);
+}
+
+// ---- soup_potential_at_noalloc: the same evaluation in a fixed device array ---------------------
+// Every buffer soup_potential_at allocates on first use lives here instead, carved at 256-byte
+// boundaries: positions (16 B), cell and order (4 B each) and partial sums (32 B) per particle up to
+// SOUP_NOALLOC_MAX_N, count and start arrays of SOUP_NOALLOC_MAX_CELLS + 1 entries, and the totals.
+// One array per GPU, held until the process ends; calls on one GPU share it.
+static constexpr size_t pa_align(size_t x) { return (x + 255u) & ~(size_t)255u; }
+static constexpr size_t kNaXs = 0;
+static constexpr size_t kNaCell = kNaXs + pa_align((size_t)SOUP_NOALLOC_MAX_N * 16u);
+static constexpr size_t kNaOrder = kNaCell + pa_align((size_t)SOUP_NOALLOC_MAX_N * 4u);
+static constexpr size_t kNaCount = kNaOrder + pa_align((size_t)SOUP_NOALLOC_MAX_N * 4u);
+static constexpr size_t kNaStart = kNaCount + pa_align(((size_t)SOUP_NOALLOC_MAX_CELLS + 1u) * 4u);
+static constexpr size_t kNaPart = kNaStart + pa_align(((size_t)SOUP_NOALLOC_MAX_CELLS + 1u) * 4u);
+static constexpr size_t kNaOut = kNaPart + pa_align((size_t)SOUP_NOALLOC_MAX_N * 32u);
+static constexpr size_t kNaBytes = kNaOut + 256u;
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+#define SOUP_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[SOUP_MAX_DEVICES];  // device address of `scratch` per GPU, once resolved
+
+// The device address of this GPU's `scratch` (the array's name is a host-side stand-in that no kernel
+// may be given): resolved once per device with that device current, then kept.
+static int scratch_on(SoupHandle* h, unsigned char** out) {
+  if (h->device < 0 || h->device >= SOUP_MAX_DEVICES)
+    return fail(h, SOUP_E_INVALID, fmt("soup_potential_at_noalloc: device %d is outside the %d this library tracks", h->device,
+                                       SOUP_MAX_DEVICES));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[h->device]) {
+    int cur = 0;
+    HC(h, hipGetDevice(&cur));
+    if (cur != h->device) HC(h, hipSetDevice(h->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != h->device) HC(h, hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(h, SOUP_E_DEVICE, std::string("soup_potential_at_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[h->device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[h->device];
+  return 0;
+}
+
+int soup_potential_at_noalloc(SoupHandle* h, const SoupPotentialBasis* pb, const float* positions, const double box[3],
+                              double out[5]) {
+  if (!h) return SOUP_E_INVALID;
+  PaGrid g;
+  size_t ncell = 0;
+  int r = pa_grid(h, "soup_potential_at_noalloc", pb, positions, box, out, g, ncell);
+  if (r) return r;
+  if (h->n > SOUP_NOALLOC_MAX_N)
+    return fail(h, SOUP_E_INVALID,
+                fmt("soup_potential_at_noalloc: %u particles exceed the fixed scratch array's %u (use soup_potential_at)", h->n,
+                    (unsigned)SOUP_NOALLOC_MAX_N));
+  if (ncell > SOUP_NOALLOC_MAX_CELLS)
+    return fail(h, SOUP_E_INVALID,
+                fmt("soup_potential_at_noalloc: the configuration's grid of %zu cells exceeds the fixed scratch array's %u (use "
+                    "soup_potential_at)",
+                    ncell, (unsigned)SOUP_NOALLOC_MAX_CELLS));
+  unsigned char* base = nullptr;
+  r = scratch_on(h, &base);
+  if (r) return r;
+  r = sync(h);
+  if (r) return r;
+  PaBufs q{reinterpret_cast<float4*>(base + kNaXs), reinterpret_cast<uint32_t*>(base + kNaCell),
+           reinterpret_cast<uint32_t*>(base + kNaOrder), reinterpret_cast<uint32_t*>(base + kNaCount),
+           reinterpret_cast<uint32_t*>(base + kNaStart), reinterpret_cast<double4*>(base + kNaPart),
+           reinterpret_cast<double*>(base + kNaOut)};
+  return pa_run(h, pb, positions, g, ncell, q, out);
 }
 
 int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positions, uint32_t flags) {

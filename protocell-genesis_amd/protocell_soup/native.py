@@ -93,12 +93,13 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsoup_hip.so")
 
 SET_BOX_FORCE = 1  # soup_set_box: run a force pass after the rebuild
+NOALLOC_MAX_N, NOALLOC_MAX_CELLS = 65536, 262144  # soup_potential_at_noalloc limits
 
 # every entry point include/soup_abi.h declares
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_reset_particles", "soup_sample_insertion", "soup_force_candidate_stats", "soup_step_phases", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_potential_at_noalloc", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
     "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_use_graphs", "soup_version",
 )
@@ -148,6 +149,8 @@ def load_library(path: Optional[str] = None) -> C.CDLL:
         "soup_potential": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(C.c_double)]),
         "soup_potential_at": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),
+        "soup_potential_at_noalloc": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double)]),
         "soup_set_box": (C.c_int, [H, C.POINTER(SoupConfig), C.c_void_p, C.c_uint32]),
         "soup_energy_adjoint": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
