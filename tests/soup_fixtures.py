@@ -39,3 +39,18 @@ def force_rel_err(f32, f64):
     mag = np.abs(f64).max(axis=1)
     den = np.maximum(mag, np.median(mag))
     return float((np.abs(f32 - f64).max(axis=1) / den).max())
+
+
+def state_of(s):
+    """Every per-particle array and counter a step advances, in particle order."""
+    rng = s.rngState()
+    return {"step": np.array([s.steps]), "pos": s.particles(), "vel": s.velocities(), "force": s.forcesNoRebuildDEBUG(),
+            "bonds": s.bondSlots(), "clink": s.centerLinks(), "cheld": s.centerHeldSteps(), "brng": rng["bond"],
+            "trng": rng["thermo"], "events": s.readDEBUG("EVENTS"), "charge": s.charges(),
+            "prng": np.array([s.protonationRngState()])}
+
+
+def assert_same_state(a, b, what):
+    sa, sb = state_of(a), state_of(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), f"{what}: {k} differs"

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import protocell_soup as ps
-from soup_fixtures import BROTH, charged
+from soup_fixtures import BROTH, assert_same_state, charged
 
 pytestmark = pytest.mark.gpu
 
@@ -23,18 +23,7 @@ def _charged(box, seed, pH, ionic):
     return o
 
 
-def _state(s):
-    rng = s.rngState()
-    return {"step": np.array([s.steps]), "pos": s.particles(), "vel": s.velocities(), "force": s.forcesNoRebuildDEBUG(),
-            "bonds": s.bondSlots(), "clink": s.centerLinks(), "cheld": s.centerHeldSteps(), "brng": rng["bond"],
-            "trng": rng["thermo"], "events": s.readDEBUG("EVENTS"), "charge": s.charges(),
-            "prng": np.array([s.protonationRngState()])}
-
-
-def _assert_same(a, b, what):
-    sa, sb = _state(a), _state(b)
-    for k in sa:
-        assert np.array_equal(sa[k], sb[k]), f"{what}: {k} differs"
+_assert_same = assert_same_state
 
 
 def _pair(opts):
