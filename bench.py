@@ -95,6 +95,41 @@ def replica_sweep(ps, torch, count, steps):
                         "charged heads with long-range DH, bond MC/20, rebuild/10, pH sweep/1000"}
 
 
+def drift_schedule(ps, torch, cadence_sys, steps):
+    """The campaign under both Verlet rebuild schedules over the same number of steps (graphs on, the
+    constant-pH sweep that falls inside included): the reference's fixed cadence on the benchmark's
+    own system, and the displacement-triggered schedule on a second copy of the campaign."""
+    r0 = cadence_sys.rebuildCount()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cadence_sys.step(steps)
+    torch.cuda.synchronize()
+    t_cad = time.perf_counter() - t
+    rc = cadence_sys.rebuildCount() - r0
+    opts = dict(CAMPAIGN, verletOverride={"rebuild": "drift"})
+    d = ps.createSoup(opts)
+    try:
+        d.relaxColdStart()
+        d.step(200)
+        r1 = d.rebuildCount()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        d.step(steps)
+        torch.cuda.synchronize()
+        t_drift = time.perf_counter() - t
+        rd = d.rebuildCount() - r1
+        reads = d.readDEBUG("LIST_READS")
+        cfg = d.verletConfig()
+    finally:
+        d.dispose()
+    log(f"rebuild schedules over {steps} steps: cadence {steps / t_cad:.0f} steps/s with {rc} rebuilds, drift "
+        f"{steps / t_drift:.0f} steps/s with {rd} rebuilds")
+    return {"steps": steps, "cadence_steps_per_s": round(steps / t_cad, 1), "cadence_rebuilds": rc,
+            "drift_steps_per_s": round(steps / t_drift, 1), "drift_rebuilds": rd,
+            "drift_check_every": cfg["driftCheckEvery"], "drift_trigger": round(cfg["driftTrigger"], 4),
+            "drift_whole_row_force_fraction": round(float(reads[1]) / max(1, int(reads.sum())), 3)}
+
+
 def cpu_baseline(sys_, R, sample_steps):
     """The oracle (C, OpenMP) stepping the same state on the host cores, plus one oracle snapshot
     analysis, both amortised to the campaign's cadence."""
@@ -187,6 +222,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--replicas", type=int, default=64, help="replicas of the batched sweep (0: skip it)")
     ap.add_argument("--replica-steps", type=int, default=2000)
+    ap.add_argument("--schedule-steps", type=int, default=1000,
+                    help="steps of the cadence-vs-drift rebuild schedule comparison (0: skip it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the FETCH_SIZE/WRITE_SIZE scale reference) and exit")
     args = ap.parse_args()
@@ -310,6 +347,8 @@ def main():
             tj = json.load(fh)
         traffic = round((1.0 - frac_whole) * tj["hbm_bytes_per_launch_near_rows"] + frac_whole * tj["hbm_bytes_per_launch_whole_rows"])
 
+    schedules = drift_schedule(ps, torch, s, args.schedule_steps) if args.schedule_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -368,6 +407,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "rebuild_schedules": schedules,
     }
     print(json.dumps(out), flush=True)
 

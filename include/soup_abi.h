@@ -89,8 +89,19 @@ typedef struct SoupConfig {
   float bond_interval_f;      /* adsorptionParams.y */
   uint32_t bond_interval;     /* bondAttemptInterval.steps */
   uint32_t capacity;          /* particles the buffers hold (the creation census; 0: n) */
-  uint32_t reserved[6];
+  /* Verlet rebuild schedule. SOUP_REBUILD_CADENCE: every rebuild_every steps, as the reference
+   * (soup/src/soup-integrate.ts:149), and the only mode the reference has. SOUP_REBUILD_DRIFT: the
+   * measured drift maximum since the last rebuild (soup/wgsl/verlet.wgsl:115-123) is read back every
+   * drift_check_every steps and the next step rebuilds once it exceeds drift_trigger (<= skin/2), so
+   * a slow system rebuilds rarely and a fast one often; lists, and the bond Monte Carlo candidates
+   * they feed, then differ from the reference's on the steps where the two schedules differ. */
+  uint32_t rebuild_mode;
+  uint32_t drift_check_every;
+  float drift_trigger;
+  uint32_t reserved[3];
 } SoupConfig;
+
+enum { SOUP_REBUILD_CADENCE = 0, SOUP_REBUILD_DRIFT = 1 };
 
 /* Mutable state at creation: a fresh lattice built by the host (soup/src/soup-init-state.ts:58-307)
  * or a decoded checkpoint (soup/src/checkpoint.ts:220-237). All arrays have n (or 3n/4n) entries and
@@ -141,8 +152,10 @@ enum {
   SOUP_FIELD_VERLET_NEAR = 21, /* u32 per particle: entries of the stored list row closer than the near/far
                                   split at the rebuild (no counterpart in the reference, whose row is one
                                   walk-ordered run; SOUP_FIELD_VERLET_LIST returns that run) */
-  SOUP_FIELD_LIST_READS = 22   /* u32 x2 since creation: list-force evaluations that read the near parts
+  SOUP_FIELD_LIST_READS = 22,  /* u32 x2 since creation: list-force evaluations that read the near parts
                                   only, and those that read whole rows (no counterpart in the reference) */
+  SOUP_FIELD_REBUILDS = 23     /* u64: Verlet rebuilds made by steps since creation, read-only (no counterpart
+                                  in the reference) */
 };
 
 /* Modes for soup_forces (soup/src/soup-readback.ts:32-65). */

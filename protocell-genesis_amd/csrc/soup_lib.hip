@@ -70,7 +70,13 @@ struct SoupHandle {
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
     float4* key = nullptr;  // b.pos when captured
-  } graphs[2];
+    uint32_t len = 0;       // steps
+    uint32_t phase = 0;     // global_step % bondAttemptInterval at the first step
+  };
+  std::vector<StepGraph> graphs;
+  // drift-triggered rebuilds (SOUP_REBUILD_DRIFT): the next step rebuilds; steps since the last check
+  bool rebuild_due = false;
+  uint64_t rebuilds = 0;  // list rebuilds made by steps (the priming and forced ones not counted)
   bool use_graphs = true;
 };
 
@@ -298,9 +304,20 @@ static void ph_bonds(SoupHandle* h) {
 
 // thermo_in: the previous step left its thermostat half to this step's first kernel; thermo_out:
 // this step leaves its own to the next (steps that follow each other inside a replayed period)
+static bool drift_mode(const SoupHandle* h) { return h->c.rebuild_mode == SOUP_REBUILD_DRIFT; }
+// whether the next step rebuilds: every rebuildEvery steps (soup/src/soup-integrate.ts:149), or, in
+// drift mode, when the last drift check asked for it
+static bool rebuild_next(const SoupHandle* h) {
+  return drift_mode(h) ? h->rebuild_due : (h->global_step % h->c.rebuild_every) == 0;
+}
+
 static void one_step(SoupHandle* h, bool thermo_in = false, bool thermo_out = false) {
   const bool do_bonds = (h->global_step % h->c.bond_interval) == 0;
-  const bool do_rebuild = (h->global_step % h->c.rebuild_every) == 0;
+  const bool do_rebuild = rebuild_next(h);
+  if (do_rebuild) {
+    h->rebuilds++;
+    h->rebuild_due = false;
+  }
   // also the drift max of this step's positions (soup-integrate.ts:96-98)
   if (thermo_in) ph_thermo_drift(h);
   else ph_kick_drift(h);
@@ -322,29 +339,30 @@ static uint32_t step_period(const SoupConfig& c) {
   return q <= 1000u ? (uint32_t)q : 0u;  // 0: no replay (a period longer than a chunk)
 }
 static void drop_graphs(SoupHandle* h) {
-  for (auto& g : h->graphs) {
+  for (auto& g : h->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    g = SoupHandle::StepGraph{};
-  }
+  h->graphs.clear();
 }
-// one period of steps through the graph captured for the current buffer assignment (captured and
-// instantiated on first use); false if capture is unavailable, with the host state untouched
+// P steps through the graph captured for the current buffer assignment and bond phase (captured and
+// instantiated on first use); false if capture is unavailable, with the host state untouched. A
+// cadence period holds an even number of rebuilds, so the buffer assignment it ends with is the one
+// it began with; a drift-mode segment holds none.
 static bool run_period(SoupHandle* h, uint32_t P) {
-  SoupHandle::StepGraph* g = nullptr;
+  const uint32_t phase = (uint32_t)(h->global_step % h->c.bond_interval);
+  const uint64_t rebuilds = drift_mode(h) ? 0u : P / h->c.rebuild_every;
   for (auto& s : h->graphs)
-    if (s.exec && s.key == h->b.pos) g = &s;
-  if (g) {
-    if (hipGraphLaunch(g->exec, h->stream) != hipSuccess) return false;
-    h->global_step += P;  // the host side of those steps: an even number of buffer swaps undoes itself
-    return true;
-  }
-  g = h->graphs[0].exec ? &h->graphs[1] : &h->graphs[0];
-  if (g->exec) {
-    (void)hipGraphExecDestroy(g->exec);
-    *g = SoupHandle::StepGraph{};
+    if (s.exec && s.key == h->b.pos && s.len == P && s.phase == phase) {
+      if (hipGraphLaunch(s.exec, h->stream) != hipSuccess) return false;
+      h->global_step += P;  // the host side of those steps: an even number of buffer swaps undoes itself
+      h->rebuilds += rebuilds;
+      return true;
+    }
+  if (h->graphs.size() >= 48) {  // a bounded cache: the oldest capture goes
+    (void)hipGraphExecDestroy(h->graphs.front().exec);
+    h->graphs.erase(h->graphs.begin());
   }
   float4* key = h->b.pos;
-  const uint64_t step0 = h->global_step;
+  const uint64_t step0 = h->global_step, rebuilds0 = h->rebuilds;
   const DevBuf b0 = h->b, a0 = h->alt;
   if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
   for (uint32_t k = 0; k < P; ++k) one_step(h, k > 0, k + 1 < P);
@@ -355,6 +373,7 @@ static bool run_period(SoupHandle* h, uint32_t P) {
   if (graph) (void)hipGraphDestroy(graph);
   // nothing ran while capturing: restore the host state, then run the period from the graph
   h->global_step = step0;
+  h->rebuilds = rebuilds0;
   h->b = b0;
   h->alt = a0;
   (void)hipGetLastError();
@@ -363,10 +382,15 @@ static bool run_period(SoupHandle* h, uint32_t P) {
     h->use_graphs = false;
     return false;
   }
-  g->exec = exec;
-  g->key = key;
+  SoupHandle::StepGraph g;
+  g.exec = exec;
+  g.key = key;
+  g.len = P;
+  g.phase = phase;
+  h->graphs.push_back(g);
   if (hipGraphLaunch(exec, h->stream) != hipSuccess) return false;
   h->global_step += P;
+  h->rebuilds += rebuilds;
   return true;
 }
 
@@ -508,6 +532,14 @@ static int validate(const SoupConfig* c) {
                 fmt("createSoup: the live composition (%u) is larger than the starting one (%u) -- buffers are allocated for the "
                     "starting one",
                     c->n, c->capacity));
+  if (c->rebuild_mode > SOUP_REBUILD_DRIFT)
+    return fail(nullptr, SOUP_E_INVALID, fmt("soup_create: unknown rebuild mode %u", c->rebuild_mode));
+  if (c->rebuild_mode == SOUP_REBUILD_DRIFT &&
+      (c->drift_check_every == 0 || c->drift_check_every > 1000u || !(c->drift_trigger > 0.0f) ||
+       !(c->drift_trigger <= 0.5f * c->skin)))
+    return fail(nullptr, SOUP_E_INVALID,
+                fmt("verletList drift mode: driftCheckEvery=%u must be 1..1000 and driftTrigger=%g must lie in (0, skin/2=%g]",
+                    c->drift_check_every, (double)c->drift_trigger, 0.5 * c->skin));
   if (c->list_cap == 0 || c->rebuild_every == 0 || c->bond_interval == 0)
     return fail(nullptr, SOUP_E_INVALID, "soup_create: list capacity, rebuildEvery and bondAttemptInterval must be > 0");
   if (c->es_enabled && c->n_heads > 0 && c->es_cap == 0)
@@ -838,22 +870,63 @@ static int chunk_end(SoupHandle* h, uint64_t chunk_start) {
   return 0;
 }
 
+// Drift mode: the measured drift maximum since the last rebuild (flags[2]) read back after a segment
+// of steps; the next step rebuilds once it exceeds the trigger, which leaves a segment's worth of
+// margin below skin/2 for the steps that run before the next check. A non-finite drift never
+// triggers: the chunk-end guards report that state.
+static int drift_check(SoupHandle* h) {
+  uint32_t bits = 0;
+  HC(h, hipMemcpyAsync(&bits, h->b.flags + 2, 4, hipMemcpyDeviceToHost, h->stream));
+  int r = sync(h);
+  if (r) return r;
+  float d2;
+  memcpy(&d2, &bits, 4);
+  if (std::sqrt(std::fmax(0.0f, d2)) > h->c.drift_trigger) h->rebuild_due = true;
+  return 0;
+}
+
+// `count` steps of one chunk. Cadence mode: whole step periods replayed from graphs where they fit.
+// Drift mode: a rebuild step when one is due, else a segment of driftCheckEvery steps without a
+// rebuild (replayed from a graph when whole), each segment followed by a drift check.
+static int chunk_steps(SoupHandle* h, uint32_t count) {
+  if (drift_mode(h)) {
+    const uint32_t E = h->c.drift_check_every;
+    for (uint32_t k = 0; k < count;) {
+      if (h->rebuild_due) {
+        one_step(h);
+        ++k;
+        continue;
+      }
+      const uint32_t seg = std::min(E, count - k);
+      if (!(seg == E && replay_ok(h) && run_period(h, E)))
+        for (uint32_t j = 0; j < seg; ++j) one_step(h);
+      k += seg;
+      const int r = drift_check(h);
+      if (r) return r;
+    }
+    return 0;
+  }
+  const uint32_t P = replay_ok(h) ? step_period(h->c) : 0u;
+  for (uint32_t k = 0; k < count;) {
+    if (P && h->global_step % P == 0 && count - k >= P && run_period(h, P)) {
+      k += P;
+      continue;
+    }
+    one_step(h);
+    ++k;
+  }
+  return 0;
+}
+
 int soup_step(SoupHandle* h, uint32_t nsteps) {
   if (!h) return SOUP_E_INVALID;
   uint32_t done = 0;
   while (done < nsteps) {
     uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
     uint64_t chunk_start = h->global_step;
-    const uint32_t P = replay_ok(h) ? step_period(h->c) : 0u;
-    for (uint32_t k = 0; k < chunk;) {
-      if (P && h->global_step % P == 0 && chunk - k >= P && run_period(h, P)) {
-        k += P;
-        continue;
-      }
-      one_step(h);
-      ++k;
-    }
-    int r = chunk_end(h, chunk_start);
+    int r = chunk_steps(h, chunk);
+    if (r) return r;
+    r = chunk_end(h, chunk_start);
     if (r) return r;
     done += chunk;
   }
@@ -870,7 +943,7 @@ int soup_step_record(SoupHandle* h, uint32_t nsteps, float* pos, uint32_t* bonds
     const uint64_t chunk_start = h->global_step;
     for (uint32_t k = 0; k < chunk; ++k, ++s) {
       one_step(h);
-      int r = 0;
+      int r = drift_mode(h) ? drift_check(h) : 0;
       if (pos) r = soup_read(h, SOUP_FIELD_POS, pos + (size_t)s * 4 * n, n * 16);
       if (!r && bonds) r = soup_read(h, SOUP_FIELD_BOND_SLOTS, bonds + (size_t)s * 3 * n, n * 12);
       if (!r && links) r = soup_read(h, SOUP_FIELD_CENTER_LINK, links + (size_t)s * n, n * 4);
@@ -907,11 +980,22 @@ int soup_set_params(SoupHandle* h, const SoupConfig* c) {
   for (int a = 0; a < 3; ++a) same = same && c->box[a] == o.box[a] && c->dims[a] == o.dims[a];
   for (uint32_t k = 0; k < SOUP_MAX_KINDS; ++k) same = same && c->radius[k] == o.radius[k] && c->cls[k] == o.cls[k];
   same = same && c->sigma == o.sigma && c->wca_factor == o.wca_factor && c->b_tt == o.b_tt && c->wc == o.wc;
+  // the integrator step and the bonded geometry (the skin basis of the lists rests on dt), the bond
+  // rules' identities, roles and bookkeeping, the sweep cadence and the rebuild schedule
+  same = same && c->dt == o.dt && c->k_fene == o.k_fene && c->r_inf == o.r_inf && c->k_bend == o.k_bend &&
+         c->r_bend == o.r_bend && c->n_rules == o.n_rules && c->catalyst_kind == o.catalyst_kind &&
+         c->head_terminal_only == o.head_terminal_only && c->head_chain_cap == o.head_chain_cap &&
+         c->max_hold_steps == o.max_hold_steps && c->bond_interval_f == o.bond_interval_f && c->sweep_every == o.sweep_every &&
+         c->rebuild_mode == o.rebuild_mode && c->drift_check_every == o.drift_check_every && c->drift_trigger == o.drift_trigger;
+  for (uint32_t k = 0; k < SOUP_MAX_RULES; ++k)
+    same = same && c->kindA[k] == o.kindA[k] && c->kindB[k] == o.kindB[k] && c->roleA[k] == o.roleA[k] &&
+           c->roleB[k] == o.roleB[k] && c->req_cat[k] == o.req_cat[k];
   if (!same)
     return fail(h, SOUP_E_INVALID,
                 "soup_set_params: only interaction strengths may change on a live system (epsilon, the attraction and acid-soap "
-                "depths, the DH coefficient, kappa and its shift force, the thermostat and bond rates); geometry, list ranges, "
-                "radii and capacities are fixed at creation");
+                "depths, the DH coefficient, kappa and its shift force, the constant-pH energies, the thermostat and the bond "
+                "attempt and accept probabilities); geometry, dt, bonded constants, list ranges, radii, capacities, bond rule "
+                "identities and roles, and the rebuild and sweep schedules are fixed at creation");
   int r = sync(h);
   if (r) return r;
   drop_graphs(h);  // their kernels carry the old parameters
@@ -946,6 +1030,15 @@ int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t nsteps, uint
         const uint32_t P = period[s];
         if (left[s] == 0) continue;
         busy = true;
+        if (drift_mode(h)) {  // its drift checks synchronise its stream: stepped through on its own
+          const int r = chunk_steps(h, left[s]);
+          left[s] = 0;
+          if (r) {
+            *failed = s;
+            return r;
+          }
+          continue;
+        }
         if (P && h->global_step % P == 0 && left[s] >= P && run_period(h, P)) {
           left[s] -= P;
           continue;
@@ -994,6 +1087,7 @@ size_t soup_field_bytes(const SoupHandle* h, int field) {
     case SOUP_FIELD_CELL_START: return ((size_t)h->ncells + 1) * 4;
     case SOUP_FIELD_MAX_DRIFT_SQ: return 4;
     case SOUP_FIELD_LIST_READS: return 8;
+    case SOUP_FIELD_REBUILDS: return 8;
     case SOUP_FIELD_HEAD_IDX: return (size_t)h->nh * 4;
     default: return 0;
   }
@@ -1006,6 +1100,10 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
     return fail(h, SOUP_E_INVALID, fmt("soup_read: unknown field %d", field));
   if (bytes != need) return fail(h, SOUP_E_INVALID, fmt("soup_read: field %d needs %zu bytes, got %zu", field, need, bytes));
   if (need == 0) return 0;
+  if (field == SOUP_FIELD_REBUILDS) {  // a host counter
+    memcpy(dst, &h->rebuilds, 8);
+    return 0;
+  }
   const void* src = h->stage;
   const uint32_t n = h->n;
   switch (field) {

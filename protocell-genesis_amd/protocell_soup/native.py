@@ -20,9 +20,10 @@ FIELD = {
     "POS": 0, "VEL": 1, "FORCE": 2, "BOND_SLOTS": 3, "CENTER_LINK": 4, "CENTER_HELD": 5, "FROZEN": 6, "CHARGE": 7,
     "BOND_RNG": 8, "THERMO_RNG": 9, "EVENTS": 10, "DESORB": 11, "VERLET_COUNT": 12, "VERLET_LIST": 13, "ES_COUNT": 14,
     "ES_LIST": 15, "CELL_OF": 16, "CELL_START": 17, "CELL_IDX": 18, "MAX_DRIFT_SQ": 19, "HEAD_IDX": 20,
-    "VERLET_NEAR": 21, "LIST_READS": 22,
+    "VERLET_NEAR": 21, "LIST_READS": 22, "REBUILDS": 23,
 }
 FORCES_REBUILD, FORCES_BRUTE, FORCES_LIST = 0, 1, 2
+REBUILD_CADENCE, REBUILD_DRIFT = 0, 1
 PHASE = {"KICK_DRIFT": 1, "REBUILD": 2, "FORCE": 4, "DRIFT": 8, "BONDS": 16, "KICK_THERMO": 32}
 
 
@@ -49,7 +50,8 @@ class SoupConfig(C.Structure):
         ("roleB", C.c_float * MAX_RULES),
         ("catalyst_kind", C.c_float), ("head_terminal_only", C.c_float), ("head_chain_cap", C.c_float),
         ("max_hold_steps", C.c_float), ("bond_interval_f", C.c_float), ("bond_interval", C.c_uint32),
-        ("capacity", C.c_uint32), ("reserved", C.c_uint32 * 6),
+        ("capacity", C.c_uint32), ("rebuild_mode", C.c_uint32), ("drift_check_every", C.c_uint32),
+        ("drift_trigger", C.c_float), ("reserved", C.c_uint32 * 3),
     ]
 
 

@@ -117,6 +117,7 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     if vo.get("enabled") is False:
         raise ValueError("createSoup: the list-free cell-walk force path is not part of this build (verletList.enabled must stay true)")
     list_cap = int(vo.get("listCapacity", cap["capacity"]))
+    rebuild = resolveRebuildSchedule(soup, p, opts["kT"], vo)
     es_over = dict(opts.get("electrostatics") or {})
     es_max_heads = int(start.get(soup["electrostatics"]["chargedKind"], 0)) if soup.get("electrostatics") else 0
     es = makeEsBasis(soup, p, {**es_over, "minBoxSigma": min_box, "maxHeads": es_max_heads,
@@ -159,12 +160,39 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
     R.opts = dict(opts)
     R.init, R.charges0, R.prng, R.events0, R.initial_step = init, charges0, prng_state, events0, initial_step
     R.next_sweep_at = initial_step + sweep_every if es.enabled else 2 ** 64 - 1
-    R.verlet = dict(soup["verletList"], listCapacity=list_cap)
+    R.verlet = dict(soup["verletList"], listCapacity=list_cap, rebuild=rebuild["mode"])
+    if rebuild["mode"] == "drift":
+        R.verlet.update(driftCheckEvery=rebuild["checkEvery"], driftTrigger=rebuild["trigger"])
+    R.rebuild = rebuild
     R.cycle_cfg, R.dry_box = cycle_cfg, dry_box
     ph = cyclePhaseAt(initial_step, cycle_cfg) if cycle_cfg else {"phase": "none", "cycleIndex": 0}
     R.cycle_phase, R.cycle_index = ph["phase"], ph["cycleIndex"]
     R.cfg = build_config(R, opts)
     return R
+
+
+def resolveRebuildSchedule(soup: dict, p: dict, kT: float, vo: Dict[str, Any]) -> Dict[str, Any]:
+    """The Verlet rebuild schedule: the reference's fixed cadence (verletList.rebuildEvery,
+    soup/src/soup-integrate.ts:149), or, with verletOverride.rebuild = "drift", a rebuild whenever the
+    measured drift maximum exceeds a trigger, checked every driftCheckEvery steps. The default trigger
+    leaves the reference's own per-step displacement estimate, dt * 2 sqrt(3 kT) (the drift condition
+    of data/soup.json's verletList basis), for every step until the next check: skin/2 - checkEvery *
+    dt * 2 sqrt(3 kT)."""
+    mode = vo.get("rebuild", "cadence")
+    if mode not in ("cadence", "drift"):
+        raise ValueError(f"verletOverride.rebuild={mode!r}: the schedule is 'cadence' (the reference's) or 'drift'")
+    if mode == "cadence":
+        return {"mode": "cadence"}
+    skin = soup["verletList"]["skin"]
+    every = int(vo.get("driftCheckEvery", 5))
+    if not 1 <= every <= 1000:
+        raise ValueError(f"verletOverride.driftCheckEvery={every} must be 1..1000")
+    v_bound = 2.0 * math.sqrt(3.0 * kT)
+    trigger = float(vo.get("driftTrigger", skin / 2 - every * p["integrator"]["dt"] * v_bound))
+    if not 0.0 < trigger <= skin / 2:
+        raise ValueError(f"verletOverride.driftTrigger={trigger:.4f} must lie in (0, skin/2={skin / 2}] "
+                         f"(at driftCheckEvery={every} the default margin leaves none)")
+    return {"mode": "drift", "checkEvery": every, "trigger": trigger}
 
 
 def build_config(R: Resolved, opts: Dict[str, Any]) -> native.SoupConfig:
@@ -223,6 +251,10 @@ def build_config(R: Resolved, opts: Dict[str, Any]) -> native.SoupConfig:
     c.max_hold_steps = soup["adsorption"]["maxHoldSteps"]
     c.bond_interval_f = interval
     c.bond_interval = interval
+    if R.rebuild["mode"] == "drift":
+        c.rebuild_mode = native.REBUILD_DRIFT
+        c.drift_check_every = R.rebuild["checkEvery"]
+        c.drift_trigger = R.rebuild["trigger"]
     return c
 
 
@@ -445,6 +477,10 @@ class SoupSystem:
         return {"verletMean": s.verlet_mean, "verletMax": s.verlet_max, "verletCap": s.verlet_cap,
                 "esMean": s.es_mean, "esMax": s.es_max, "esCap": s.es_cap,
                 "verletPairs": int(s.verlet_pairs), "esPairs": int(s.es_pairs)}
+
+    def rebuildCount(self) -> int:
+        """Verlet rebuilds the steps of this system made since creation (forced ones not counted)."""
+        return int(self._read("REBUILDS", np.uint64)[0])
 
     def verletList(self) -> np.ndarray:
         return self._read("VERLET_LIST", np.uint32).reshape(self.N, self._R.list_cap)
