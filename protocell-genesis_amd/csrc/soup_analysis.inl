@@ -44,9 +44,19 @@ __global__ void k_uf_init(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
 }
+// The final labels: every entry set to its root. The walk here stores nothing along the way: a
+// path-splitting store of one thread could land on an entry after that entry's own thread wrote its
+// root there, and leave it pointing at a non-root ancestor. With only each thread's own final store,
+// every value a walk reads is an ancestor, so every walk ends at the root.
 __global__ void k_uf_flatten(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = uf_root(p, i);
+  if (i >= n) return;
+  uint32_t cur = i, nxt = uf_ld(p, cur);
+  while (nxt != cur) {
+    cur = nxt;
+    nxt = uf_ld(p, cur);
+  }
+  __hip_atomic_store(p + i, cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double an_round(double x) {  // Math.round

@@ -92,17 +92,18 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
   return base + x - v;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* in, uint32_t m, uint32_t* block_sums) {
+__device__ __forceinline__ void scan_reduce_body(const uint32_t* in, uint32_t m, uint32_t* block_sums, const uint32_t blk) {
   __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
-  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v = 0;
+  uint32_t base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k)
     if (base + k < m) v += in[base + k];
   uint32_t total;
   (void)block_excl_scan(v, s_wave, total);
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) block_sums[blk] = total;
 }
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t* in, uint32_t m, uint32_t* block_sums) { scan_reduce_body(in, m, block_sums, blockIdx.x); }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_blocks(uint32_t* block_sums, uint32_t nb) {
+__device__ __forceinline__ void scan_blocks_body(uint32_t* block_sums, uint32_t nb, const uint32_t blk) {
   // single block, exclusive scan in place over nb <= SCAN_TILE entries; block_sums[nb] = total
   __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
   uint32_t base = threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
@@ -118,23 +119,25 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_blocks(uint32_t* block_sums
   }
   if (threadIdx.x == 0) block_sums[nb] = total;
 }
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_blocks(uint32_t* block_sums, uint32_t nb) { scan_blocks_body(block_sums, nb, blockIdx.x); }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint32_t* in, uint32_t m, const uint32_t* block_sums,
-                                                           uint32_t nb, uint32_t* out) {
+__device__ __forceinline__ void scan_apply_body(const uint32_t* in, uint32_t m, const uint32_t* block_sums, uint32_t nb, uint32_t* out, const uint32_t blk) {
   __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
-  uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
+  uint32_t base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     v[k] = base + k < m ? in[base + k] : 0u;
     sum += v[k];
   }
   uint32_t total;
-  uint32_t off = block_excl_scan(sum, s_wave, total) + block_sums[blockIdx.x];
+  uint32_t off = block_excl_scan(sum, s_wave, total) + block_sums[blk];
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < m) out[base + k] = off;
     off += v[k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = block_sums[nb];
+  if (blk == 0 && threadIdx.x == 0) out[m] = block_sums[nb];
 }
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_apply(const uint32_t* in, uint32_t m, const uint32_t* block_sums,
+                                                           uint32_t nb, uint32_t* out) { scan_apply_body(in, m, block_sums, nb, out, blockIdx.x); }
 
 // |minimum-image displacement since the last rebuild|^2 as bits, uncontracted (verlet.wgsl:115-123)
 __device__ __forceinline__ uint32_t drift_sq_bits(const SoupConfig& c, float4 xi, float4 x0) {
@@ -215,9 +218,9 @@ __device__ __forceinline__ void drift_max_block(DevBuf& b, uint32_t bits, uint32
   }
 }
 
-__global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
+__device__ __forceinline__ void kick_drift_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   __shared__ uint32_t s_max[256 / WAVE];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blk * blockDim.x + threadIdx.x;
   uint32_t bits = 0;
   if (t < c.n) {
     float4 x = b.pos[t];
@@ -228,9 +231,10 @@ __global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) {
   }
   drift_max_block(b, bits, s_max);
 }
+__global__ __launch_bounds__(256) void k_kick_drift(SoupConfig c, DevBuf b) { kick_drift_body(c, b, blockIdx.x); }
 
-__global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void kick_thermo_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   if (b.frozen[t] != 0u) {
     b.vel[t] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -238,14 +242,15 @@ __global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) {
   }
   b.vel[t] = kick_thermo_one(c, b, t, b.vel[t], b.force[t]);
 }
+__global__ __launch_bounds__(256) void k_kick_thermo(SoupConfig c, DevBuf b) { kick_thermo_body(c, b, blockIdx.x); }
 
 // kick_thermostat_main of the step that ended fused with kick_drift_wrap_main of the next, for two
 // steps that follow each other with nothing in between (inside a replayed step period): both read
 // the same force, and the velocity passes between them in registers, so the result is the two
 // kernels' bit for bit
-__global__ __launch_bounds__(256) void k_thermo_drift(SoupConfig c, DevBuf b) {
+__device__ __forceinline__ void thermo_drift_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   __shared__ uint32_t s_max[256 / WAVE];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t t = blk * blockDim.x + threadIdx.x;
   uint32_t bits = 0;
   if (t < c.n) {
     float4 x = b.pos[t];
@@ -260,6 +265,7 @@ __global__ __launch_bounds__(256) void k_thermo_drift(SoupConfig c, DevBuf b) {
   }
   drift_max_block(b, bits, s_max);
 }
+__global__ __launch_bounds__(256) void k_thermo_drift(SoupConfig c, DevBuf b) { thermo_drift_body(c, b, blockIdx.x); }
 
 // soup_max_drift_main (verlet.wgsl:115-123): wave max, one atomic per wave.
 __global__ __launch_bounds__(256) void k_max_drift(SoupConfig c, DevBuf b) {
@@ -307,33 +313,36 @@ __global__ __launch_bounds__(256) void k_relax_step(SoupConfig c, DevBuf b, floa
 // Rebuild: stable counting sort by (cell, particle index) + permutation of all state
 // (replaces clear/count/prefix/fill_main, engine/wgsl/neighbor.wgsl:26-64)
 // ================================================================================================
-__global__ __launch_bounds__(256) void k_cell_count(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_count_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   uint32_t cell = cell_of(c, b.pos[t]);
   b.cell_tmp[t] = cell;
   b.cell_rank[t] = atomicAdd(&b.cell_count[cell], 1u);
 }
+__global__ __launch_bounds__(256) void k_cell_count(SoupConfig c, DevBuf b) { cell_count_body(c, b, blockIdx.x); }
 
 // unordered fill into the cell segment, then rank by particle index inside the segment
-__global__ __launch_bounds__(256) void k_cell_fill(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_fill_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   b.perm[b.cell_start[b.cell_tmp[t]] + b.cell_rank[t]] = t;  // perm used as scratch here
 }
+__global__ __launch_bounds__(256) void k_cell_fill(SoupConfig c, DevBuf b) { cell_fill_body(c, b, blockIdx.x); }
 
-__global__ __launch_bounds__(256) void k_cell_rank(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_rank_body(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   uint32_t cell = b.cell_tmp[t], me = b.id[t];
   uint32_t s0 = b.cell_start[cell], s1 = b.cell_start[cell + 1], r = 0;
   for (uint32_t k = s0; k < s1; ++k) r += (b.id[b.perm[k]] < me) ? 1u : 0u;
   new_slot_of_old[t] = s0 + r;
 }
+__global__ __launch_bounds__(256) void k_cell_rank(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old) { cell_rank_body(c, b, new_slot_of_old, blockIdx.x); }
 
 // gathers every per-particle array into the new slot order (out = the alternate buffers)
-__global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void permute_body(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   uint32_t ns = new_slot_of_old[t];
   float4 x = b.pos[t];
@@ -355,9 +364,10 @@ __global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf 
   o.cell_of[ns] = b.cell_tmp[t];
   b.inv[id] = ns;
 }
+__global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old) { permute_body(c, b, o, new_slot_of_old, blockIdx.x); }
 
-__global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
-  uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void head_slots_body(DevBuf b, uint32_t nh, const uint32_t blk) {
+  uint32_t h = blk * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   uint32_t u = b.inv[b.head_id[h]];
   b.head_slot[h] = u;
@@ -366,20 +376,23 @@ __global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) {
   b.head_of_slot[u] = h;
   b.head_flag[u] = 1u;
 }
+__global__ __launch_bounds__(256) void k_head_slots(DevBuf b, uint32_t nh) { head_slots_body(b, nh, blockIdx.x); }
 
 // heads in slot order: the far-force and pair-table kernels walk rows in this spatial order
 // head_q from the current positions and charges (every force evaluation)
-__global__ __launch_bounds__(256) void k_head_refresh(DevBuf b, uint32_t nh) {
-  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void head_refresh_body(DevBuf b, uint32_t nh, const uint32_t blk) {
+  const uint32_t h = blk * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   const uint32_t u = b.head_slot[h];
   const float4 x = b.pos[u];
   b.head_q[h] = make_float4(x.x, x.y, x.z, b.charge[u]);
 }
-__global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_head_refresh(DevBuf b, uint32_t nh) { head_refresh_body(b, nh, blockIdx.x); }
+__device__ __forceinline__ void head_rows_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t < c.n && b.head_flag[t]) b.head_rows[b.head_pos[t]] = b.head_of_slot[t];
 }
+__global__ __launch_bounds__(256) void k_head_rows(SoupConfig c, DevBuf b) { head_rows_body(c, b, blockIdx.x); }
 
 // largest WCA cut of any kind pair (also the largest bond-formation and catalyst distance)
 __device__ __forceinline__ float wca_reach(const SoupConfig& c) {
@@ -424,12 +437,12 @@ __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t c
 #ifndef VL_BATCH
 #define VL_BATCH 4u  // Verlet-build candidate chunks of 64 whose loads are issued together
 #endif
-__global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
+__device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   __shared__ uint32_t s_end[256 / WAVE][WAVE];  // inclusive prefix of kept row lengths
   __shared__ uint4 s_seg[256 / WAVE][WAVE];     // first slot of range a, its length, first slot of range b
   __shared__ float4 s_sh[256 / WAVE][WAVE];     // image shifts: x of range a, x of range b, y, z
   const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
-  const uint32_t t = uni(xcd_block() * (blockDim.x / WAVE) + w);
+  const uint32_t t = uni(blk * (blockDim.x / WAVE) + w);
   if (t >= c.n) return;
   const float4 xv = b.pos[t];
   const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
@@ -558,6 +571,7 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
     if (count > cap) b.flags[0] = 1u;
   }
 }
+__global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) { verlet_build_body(c, b, xcd_block()); }
 
 // ================================================================================================
 // DH head list: soup_build_es_list_main (electrostatics-long.wgsl:31-56), one wavefront per head,
@@ -568,9 +582,9 @@ __global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) {
 #ifndef ES_BATCH
 #define ES_BATCH 8u  // DH-list candidate chunks of 64 whose loads are issued together
 #endif
-__global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) {
+__device__ __forceinline__ void es_build_body(SoupConfig c, DevBuf b, uint32_t nh, const uint32_t blk) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t h = uni((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
+  const uint32_t h = uni((blk * blockDim.x + threadIdx.x) / WAVE);
   if (h >= nh) return;
   const float4 xv = b.head_q[h];
   const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
@@ -613,6 +627,7 @@ __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32
     if (count > cap) b.flags[1] = 1u;
   }
 }
+__global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) { es_build_body(c, b, nh, blockIdx.x); }
 
 // ================================================================================================
 // Forces: soup_force_list_main (verlet.wgsl:132-149) = bondedForce (step.wgsl:152-193) + the list
@@ -877,10 +892,10 @@ __device__ __forceinline__ double group_sum_d(double v) {
 // Far DH rows of the charged heads (soup_es_force_far_main, electrostatics-long.wgsl:63-82), one
 // wavefront per head in slot order, into far_acc in FP64; the list force adds them before it rounds
 // the head's force to f32.
-__global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t nh) {
+__device__ __forceinline__ void far_dh_body(SoupConfig c, DevBuf b, uint32_t nh, const uint32_t blk) {
 #pragma clang fp contract(fast)
   const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
-  const uint32_t r = uni(xcd_block() * (blockDim.x / WAVE) + w);
+  const uint32_t r = uni(blk * (blockDim.x / WAVE) + w);
   if (r >= nh) return;
   const uint32_t hh = uni(b.head_rows[r]);
   const float4 xv = b.head_q[hh];
@@ -915,6 +930,7 @@ __global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t
     b.far_acc[3u * hh + 2u] = f.z;
   }
 }
+__global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t nh) { far_dh_body(c, b, nh, xcd_block()); }
 
 #ifndef FORCE_TPP
 #define FORCE_TPP 8u    // lanes per particle
@@ -931,12 +947,12 @@ __global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t
 // farther than reach + 0.5 sigma from every face has no pair that wraps inside the reach, so it
 // takes the plain difference (equal to the minimum image for every pair that can act); the others
 // take the exact image of mi_acc.
-__global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int es) {
+__device__ __forceinline__ void force_group_body(SoupConfig c, DevBuf b, int es, const uint32_t blk) {
 #pragma clang fp contract(fast)
   __shared__ PairTab T;
   load_pair_tab(c, T);
   const uint32_t sub = threadIdx.x & (FORCE_TPP - 1u);
-  const uint32_t g = (xcd_block() * blockDim.x + threadIdx.x) / FORCE_TPP;
+  const uint32_t g = (blk * blockDim.x + threadIdx.x) / FORCE_TPP;
   const bool valid = g < c.n;
   const uint32_t t = valid ? g : c.n - 1u;
   const PairConst P = make_pair_const(c);
@@ -995,6 +1011,7 @@ __global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int
   f.z = group_sum_d<FORCE_TPP>(f.z);
   if (valid && sub == 0u) b.force[t] = make_float4((float)f.x, (float)f.y, (float)f.z, 0.f);
 }
+__global__ __launch_bounds__(256) void k_force_group(SoupConfig c, DevBuf b, int es) { force_group_body(c, b, es, xcd_block()); }
 
 // O(N^2) reference force: soup_force_brute_main (step.wgsl:277-292), test path only
 __global__ __launch_bounds__(256) void k_force_brute(SoupConfig c, DevBuf b) {
@@ -1075,9 +1092,9 @@ __device__ __forceinline__ uint32_t role_of(const SoupConfig& c, uint32_t r, flo
 
 // one wavefront per particle: the lanes test 64 list entries at a time, and ballots keep the first
 // catalyst within reach and the first SOUP_CAND_CAP candidates in list order, as the serial walk does
-__global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
+__device__ __forceinline__ void bond_propose_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
-  const uint32_t t = xcd_block() * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+  const uint32_t t = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   if (t >= c.n) return;
   const float4 xi = b.pos[t];
   const float ti = xi.w;
@@ -1157,11 +1174,12 @@ __global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) {
     b.cat_id[t] = catId;
   }
 }
+__global__ __launch_bounds__(256) void k_bond_propose(SoupConfig c, DevBuf b) { bond_propose_body(c, b, xcd_block()); }
 
 // drops termination proposals that cannot succeed in any order (the carbon has no centre and no
 // propagation proposal can give it one), then flags the particles that carry an event.
-__global__ __launch_bounds__(256) void k_bond_filter(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bond_filter_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   const uint32_t i = b.id[t];
   const float ti = b.pos[t].w;
@@ -1183,12 +1201,14 @@ __global__ __launch_bounds__(256) void k_bond_filter(SoupConfig c, DevBuf b) {
   b.nacc[t] = keep;
   if (keep > 0u || b.clink[t] != SOUP_NONE || b.flag_cc[i] != 0u) b.ev_flag[i] = 1u;
 }
+__global__ __launch_bounds__(256) void k_bond_filter(SoupConfig c, DevBuf b) { bond_filter_body(c, b, blockIdx.x); }
 
-__global__ __launch_bounds__(256) void k_bond_events(SoupConfig c, DevBuf b) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bond_events_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= c.n) return;
   if (b.ev_flag[i]) b.ev_id[b.ev_pos[i]] = i;
 }
+__global__ __launch_bounds__(256) void k_bond_events(SoupConfig c, DevBuf b) { bond_events_body(c, b, blockIdx.x); }
 
 // ---- phase B helpers: all shared state accessed with device-scope atomics ----------------------
 __device__ __forceinline__ uint32_t ld(const uint32_t* p) {
@@ -1417,7 +1437,7 @@ __device__ __forceinline__ void phase_barrier() {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(1024) void k_bond_resolve(SoupConfig c, DevBuf b) {
+__device__ __forceinline__ void bond_resolve_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   __shared__ uint32_t s_pending;
   const uint32_t nev = b.ev_pos[c.n];
   const BondCtx X{c, b};
@@ -1474,11 +1494,12 @@ __global__ __launch_bounds__(1024) void k_bond_resolve(SoupConfig c, DevBuf b) {
     }
   }
 }
+__global__ __launch_bounds__(1024) void k_bond_resolve(SoupConfig c, DevBuf b) { bond_resolve_body(c, b, blockIdx.x); }
 
 // bond_break_main (bond-dispatch.wgsl:170-200): each particle breaks only bonds to partners >= i,
 // and releases the partner's slot holding i, which the partner's own pass never reads.
-__global__ __launch_bounds__(256) void k_bond_break(SoupConfig c, DevBuf b) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bond_break_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   const uint32_t i = b.id[t];
   const float ti = b.pos[t].w;
@@ -1502,6 +1523,7 @@ __global__ __launch_bounds__(256) void k_bond_break(SoupConfig c, DevBuf b) {
   }
   b.brng[t] = rng;
 }
+__global__ __launch_bounds__(256) void k_bond_break(SoupConfig c, DevBuf b) { bond_break_body(c, b, blockIdx.x); }
 
 // ================================================================================================
 // Constant-pH Monte Carlo (soup/src/electrostatics.ts:435-496, acid-soap.ts:170-198) on the device.

@@ -11,15 +11,20 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "soup_kernels.inl"
+#include "soup_packed.inl"
 
 #define SOUP_VERSION "libsoup_hip 0.1 (gfx950)"
 
 static thread_local std::string g_create_error;
+
+struct SoupHandle;
+static void forget_batches_of(const SoupHandle* h);
 
 // Per-phase device timing with HIP events on the handle's stream (soup_profile): every bracketed
 // launch gets an event pair from a pool; pairs are resolved at the next synchronisation.
@@ -77,6 +82,7 @@ struct SoupHandle {
   // drift-triggered rebuilds (SOUP_REBUILD_DRIFT): the next step rebuilds; steps since the last check
   bool rebuild_due = false;
   uint64_t rebuilds = 0;  // list rebuilds made by steps (the priming and forced ones not counted)
+  uint64_t gen = 0;       // bumped whenever the configuration, the particle count or a buffer changes
   bool use_graphs = true;
 };
 
@@ -342,6 +348,7 @@ static void drop_graphs(SoupHandle* h) {
   for (auto& g : h->graphs)
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   h->graphs.clear();
+  h->gen++;  // every change that voids a captured step also voids a batch's view of this system
 }
 // P steps through the graph captured for the current buffer assignment and bond phase (captured and
 // instantiated on first use); false if capture is unavailable, with the host state untouched. A
@@ -840,6 +847,7 @@ int soup_profile_read(SoupHandle* h, double* ms, uint64_t* count) {
 
 int soup_destroy(SoupHandle* h) {
   if (!h) return 0;
+  forget_batches_of(h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->prof.pending.clear();
   for (hipEvent_t e : h->prof.pool) (void)hipEventDestroy(e);
@@ -1006,10 +1014,372 @@ int soup_set_params(SoupHandle* h, const SoupConfig* c) {
   return sync(h);
 }
 
+}  // extern "C": the packed batch's C++ helpers follow
+
+// ================================================================================================
+// Packed replica steps: one launch per kernel covers every system of a batch (soup_packed.inl)
+// ================================================================================================
+// Launch shapes of the packed kernels; each has a table of block offsets per system.
+enum PackKind {
+  PK_P256 = 0,   // 256 threads per particle block
+  PK_P256_ES,    // the same, systems with the long-range DH term only
+  PK_WAVE,       // one wavefront per particle
+  PK_FORCE,      // FORCE_TPP lanes per particle
+  PK_H256,       // one thread per charged head
+  PK_HWAVE,      // one wavefront per charged head
+  PK_CELLS,      // one thread per grid cell
+  PK_SCAN_C,     // scan tiles over the cells
+  PK_SCAN_N,     // scan tiles over the particles
+  PK_SCAN_N_ES,  // the same, systems with the DH term only
+  PK_ONE,        // one workgroup per system
+  PK_ONE_ES,     // one workgroup per system with the DH term
+  PK_KINDS
+};
+
+// The device view of a batch: descriptor tables for the systems' current buffer assignment (tab[0])
+// and for the swapped one a rebuild leaves (tab[1]), the offset tables, a stream of its own and the
+// step periods captured on it. Built for one list of systems and their generations; any change to a
+// system's configuration, particle count or buffers (its generation) rebuilds it.
+struct BatchCtx {
+  std::vector<SoupHandle*> hs;
+  std::vector<uint64_t> gens;
+  std::vector<float4*> keys;  // each system's b.pos the tables were written for
+  int device = 0;
+  hipStream_t stream = nullptr;
+  RepDesc* tab[2] = {nullptr, nullptr};
+  uint32_t* off = nullptr;  // PK_KINDS x (nrep + 1)
+  uint32_t total[PK_KINDS] = {0};
+  int cur = 0;
+  bool any_es = false;
+  std::vector<SoupHandle::StepGraph> graphs;
+  ~BatchCtx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (tab[0]) (void)hipFree(tab[0]);
+    if (tab[1]) (void)hipFree(tab[1]);
+    if (off) (void)hipFree(off);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  uint32_t nrep() const { return (uint32_t)hs.size(); }
+  const uint32_t* offs(int kind) const { return off + (size_t)kind * (hs.size() + 1); }
+};
+
+static std::mutex g_batch_mu;
+static std::vector<std::unique_ptr<BatchCtx>> g_batches;  // the most recently used last
+
+// the 11 double-buffered arrays a rebuild permutes, exchanged between two DevBufs
+static void swap_permuted(DevBuf& x, DevBuf& y) {
+  std::swap(x.pos, y.pos);
+  std::swap(x.vel, y.vel);
+  std::swap(x.force, y.force);
+  std::swap(x.charge, y.charge);
+  std::swap(x.bs, y.bs);
+  std::swap(x.clink, y.clink);
+  std::swap(x.cheld, y.cheld);
+  std::swap(x.frozen, y.frozen);
+  std::swap(x.brng, y.brng);
+  std::swap(x.trng, y.trng);
+  std::swap(x.id, y.id);
+}
+
+// writes both descriptor tables for the systems' current buffer assignment
+static hipError_t pack_tables(BatchCtx& B) {
+  const size_t nr = B.hs.size();
+  std::vector<RepDesc> t0(nr), t1(nr);
+  for (size_t r = 0; r < nr; ++r) {
+    SoupHandle* h = B.hs[r];
+    RepDesc d{};
+    d.c = h->c;
+    d.b = h->b;
+    d.alt = h->alt;
+    d.nh = es_on(h) ? h->nh : 0u;
+    d.es = es_on(h) ? 1u : 0u;
+    t0[r] = d;
+    swap_permuted(d.b, d.alt);
+    t1[r] = d;
+    B.keys[r] = h->b.pos;
+  }
+  hipError_t e = hipMemcpyAsync(B.tab[0], t0.data(), nr * sizeof(RepDesc), hipMemcpyHostToDevice, B.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(B.tab[1], t1.data(), nr * sizeof(RepDesc), hipMemcpyHostToDevice, B.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(B.stream);
+  B.cur = 0;
+  return e;
+}
+
+// the batch of these systems, built (or rebuilt after a change) on demand; null with a message in
+// *why when it cannot be built
+static BatchCtx* batch_of(SoupHandle* const* hs, uint32_t count, std::string* why) {
+  std::lock_guard<std::mutex> lock(g_batch_mu);
+  for (size_t k = 0; k < g_batches.size(); ++k) {
+    BatchCtx* B = g_batches[k].get();
+    if (B->hs.size() != count || !std::equal(B->hs.begin(), B->hs.end(), hs)) continue;
+    bool same = true;
+    for (uint32_t r = 0; r < count && same; ++r) same = B->gens[r] == hs[r]->gen;
+    if (same) {
+      std::unique_ptr<BatchCtx> keep = std::move(g_batches[k]);
+      g_batches.erase(g_batches.begin() + k);
+      g_batches.push_back(std::move(keep));
+      return g_batches.back().get();
+    }
+    g_batches.erase(g_batches.begin() + k);  // stale: built for an older state of some system
+    break;
+  }
+  std::unique_ptr<BatchCtx> B(new BatchCtx());
+  B->hs.assign(hs, hs + count);
+  B->device = hs[0]->device;
+  for (uint32_t r = 0; r < count; ++r) {
+    B->gens.push_back(hs[r]->gen);
+    B->any_es = B->any_es || es_on(hs[r]);
+  }
+  B->keys.resize(count);
+  const size_t nr = count;
+  std::vector<uint32_t> off((size_t)PK_KINDS * (nr + 1), 0u);
+  for (size_t r = 0; r < nr; ++r) {
+    const SoupHandle* h = hs[r];
+    const uint64_t n = h->n, nh = es_on(h) ? h->nh : 0u, ncell = (uint64_t)h->c.dims[0] * h->c.dims[1] * h->c.dims[2];
+    const bool es = es_on(h);
+    const uint32_t tiles_n = std::max(1u, blocks(n, SCAN_TILE)), tiles_c = std::max(1u, blocks(ncell, SCAN_TILE));
+    const uint32_t nb[PK_KINDS] = {blocks(n, 256),       es ? blocks(n, 256) : 0u, blocks(n * WAVE, 256),
+                                   blocks(n * FORCE_TPP, 256), blocks(nh, 256),     blocks(nh * WAVE, 256),
+                                   blocks(ncell, 256),   tiles_c,                   tiles_n,
+                                   es ? tiles_n : 0u,    1u,                        es ? 1u : 0u};
+    for (int k = 0; k < PK_KINDS; ++k) off[(size_t)k * (nr + 1) + r + 1] = off[(size_t)k * (nr + 1) + r] + nb[k];
+  }
+  for (int k = 0; k < PK_KINDS; ++k) B->total[k] = off[(size_t)k * (nr + 1) + nr];
+  int cur_dev = 0;
+  bool ok = hipGetDevice(&cur_dev) == hipSuccess && cur_dev == B->device &&
+            hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc((void**)&B->tab[0], nr * sizeof(RepDesc)) == hipSuccess &&
+            hipMalloc((void**)&B->tab[1], nr * sizeof(RepDesc)) == hipSuccess &&
+            hipMalloc((void**)&B->off, off.size() * sizeof(uint32_t)) == hipSuccess &&
+            hipMemcpy(B->off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess &&
+            pack_tables(*B) == hipSuccess;
+  if (!ok) {
+    *why = "soup_step_batch: the batch's device tables could not be set up on the systems' device";
+    return nullptr;
+  }
+  if (g_batches.size() >= 4) g_batches.erase(g_batches.begin());
+  g_batches.push_back(std::move(B));
+  return g_batches.back().get();
+}
+
+// drops every batch that holds this system (it is being destroyed)
+static void forget_batches_of(const SoupHandle* h) {
+  std::lock_guard<std::mutex> lock(g_batch_mu);
+  for (size_t k = g_batches.size(); k-- > 0;)
+    if (std::find(g_batches[k]->hs.begin(), g_batches[k]->hs.end(), h) != g_batches[k]->hs.end())
+      g_batches.erase(g_batches.begin() + k);
+}
+
+// The packed batch path is switched off: its first run on the GPU (five uneven replicas, 230 steps)
+// ended in an illegal memory access whose cause is not yet found. soup_step_batch uses the per-system
+// round robin below until it is.
+static constexpr bool kPackedBatch = false;
+
+// whether these systems can step as one packed batch: one device, the reference's rebuild cadence,
+// the same rebuild and bond intervals and step phase, no profile running
+static bool packable(SoupHandle* const* hs, uint32_t count) {
+  if (!kPackedBatch) return false;
+  const SoupHandle* a = hs[0];
+  const uint32_t P = step_period(a->c);
+  if (!P) return false;
+  for (uint32_t r = 0; r < count; ++r) {
+    const SoupHandle* h = hs[r];
+    if (h->device != a->device || drift_mode(h) || h->prof.mask != 0 || h->c.rebuild_every != a->c.rebuild_every ||
+        h->c.bond_interval != a->c.bond_interval || h->global_step % P != a->global_step % P)
+      return false;
+    for (uint32_t q = 0; q < r; ++q)
+      if (hs[q] == h) return false;
+  }
+  return true;
+}
+
+template <typename K, typename... A>
+static void pk_launch(BatchCtx& B, K kern, int kind, uint32_t threads, A... extra) {
+  const uint32_t g = B.total[kind];
+  if (g) hipLaunchKernelGGL(kern, dim3(g), dim3(threads), 0, B.stream, B.tab[B.cur], B.offs(kind), B.nrep(), extra...);
+}
+
+static void pk_scan(BatchCtx& B, uint32_t which, int tiles, int one) {
+  pk_launch(B, kp_scan_reduce, tiles, SCAN_BLOCK, which);
+  pk_launch(B, kp_scan_blocks, one, SCAN_BLOCK, which);
+  pk_launch(B, kp_scan_apply, tiles, SCAN_BLOCK, which);
+}
+
+// ph_rebuild of every system in one set of launches
+static void pk_rebuild(BatchCtx& B) {
+  pk_launch(B, kp_clear_cells, PK_CELLS, 256);
+  pk_launch(B, kp_cell_count, PK_P256, 256);
+  pk_scan(B, 0u, PK_SCAN_C, PK_ONE);
+  pk_launch(B, kp_cell_fill, PK_P256, 256);
+  pk_launch(B, kp_cell_rank, PK_P256, 256);
+  pk_launch(B, kp_permute, PK_P256, 256);
+  B.cur ^= 1;  // every system now reads its permuted arrays
+  for (SoupHandle* h : B.hs) swap_permuted(h->b, h->alt);
+  if (B.any_es) {
+    pk_launch(B, kp_clear_heads, PK_P256_ES, 256);
+    pk_launch(B, kp_head_slots, PK_H256, 256);
+    pk_scan(B, 1u, PK_SCAN_N_ES, PK_ONE_ES);
+    pk_launch(B, kp_head_rows, PK_P256_ES, 256);
+    pk_launch(B, kp_es_build, PK_HWAVE, 256);
+  }
+  pk_launch(B, kp_verlet_build, PK_WAVE, 256);
+  hipLaunchKernelGGL(kp_clear_drift, dim3(blocks(B.nrep(), 64)), dim3(64), 0, B.stream, B.tab[B.cur], B.nrep());
+}
+
+// one_step of every system: all are at the same step phase, so the rebuild and bond decisions agree
+static void pk_step(BatchCtx& B, bool thermo_in, bool thermo_out) {
+  const SoupHandle* a = B.hs[0];
+  const bool do_bonds = (a->global_step % a->c.bond_interval) == 0;
+  const bool do_rebuild = (a->global_step % a->c.rebuild_every) == 0;
+  pk_launch(B, thermo_in ? kp_thermo_drift : kp_kick_drift, PK_P256, 256);
+  if (do_rebuild) pk_rebuild(B);
+  if (B.any_es) {
+    pk_launch(B, kp_head_refresh, PK_H256, 256);
+    pk_launch(B, kp_far_dh, PK_HWAVE, 256);
+  }
+  pk_launch(B, kp_force_group, PK_FORCE, 256);
+  if (do_bonds) {
+    pk_launch(B, kp_clear_bond_flags, PK_P256, 256);
+    pk_launch(B, kp_bond_propose, PK_WAVE, 256);
+    pk_launch(B, kp_bond_filter, PK_P256, 256);
+    pk_scan(B, 2u, PK_SCAN_N, PK_ONE);
+    pk_launch(B, kp_bond_events, PK_P256, 256);
+    pk_launch(B, kp_bond_resolve, PK_ONE, 1024);
+    pk_launch(B, kp_bond_break, PK_P256, 256);
+  }
+  if (!thermo_out) pk_launch(B, kp_kick_thermo, PK_P256, 256);
+  for (SoupHandle* h : B.hs) {
+    if (do_rebuild) h->rebuilds++;
+    h->global_step++;
+  }
+}
+
+// one period of P steps of the whole batch from a graph captured on the batch's stream (its
+// buffer assignment returns to where it began); false when capture is unavailable
+static bool pk_period(BatchCtx& B, uint32_t P) {
+  const uint32_t phase = (uint32_t)(B.hs[0]->global_step % B.hs[0]->c.bond_interval);
+  for (auto& g : B.graphs)
+    if (g.exec && g.len == P && g.phase == phase) {
+      if (hipGraphLaunch(g.exec, B.stream) != hipSuccess) return false;
+      for (SoupHandle* h : B.hs) {
+        h->global_step += P;
+        h->rebuilds += P / h->c.rebuild_every;
+      }
+      return true;
+    }
+  std::vector<uint64_t> step0, reb0;
+  std::vector<DevBuf> b0, a0;
+  for (SoupHandle* h : B.hs) {
+    step0.push_back(h->global_step);
+    reb0.push_back(h->rebuilds);
+    b0.push_back(h->b);
+    a0.push_back(h->alt);
+  }
+  const int cur0 = B.cur;
+  if (hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  for (uint32_t k = 0; k < P; ++k) pk_step(B, k > 0, k + 1 < P);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(B.stream, &graph);
+  hipGraphExec_t exec = nullptr;
+  const bool ok = ec == hipSuccess && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  for (size_t r = 0; r < B.hs.size(); ++r) {
+    B.hs[r]->global_step = step0[r];
+    B.hs[r]->rebuilds = reb0[r];
+    B.hs[r]->b = b0[r];
+    B.hs[r]->alt = a0[r];
+  }
+  B.cur = cur0;
+  (void)hipGetLastError();
+  if (!ok) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return false;
+  }
+  SoupHandle::StepGraph g;
+  g.exec = exec;
+  g.len = P;
+  g.phase = phase;
+  B.graphs.push_back(g);
+  if (hipGraphLaunch(exec, B.stream) != hipSuccess) return false;
+  for (SoupHandle* h : B.hs) {
+    h->global_step += P;
+    h->rebuilds += P / h->c.rebuild_every;
+  }
+  return true;
+}
+
+// soup_step_batch over a packed batch: each chunk's steps as packed launches (whole periods from
+// graphs when every system replays graphs), then every system's chunk-end guards and due sweep
+static int step_packed(BatchCtx& B, uint32_t nsteps, uint32_t* failed) {
+  const uint32_t count = B.nrep();
+  for (uint32_t r = 0; r < count; ++r) {  // the systems' own streams are idle before the batch writes
+    const int e = sync(B.hs[r]);
+    if (e) {
+      *failed = r;
+      return e;
+    }
+  }
+  bool keys_ok = true;
+  for (uint32_t r = 0; r < count; ++r) keys_ok = keys_ok && B.keys[r] == B.hs[r]->b.pos;
+  if (!keys_ok && pack_tables(B) != hipSuccess) {
+    *failed = 0;
+    return fail(B.hs[0], SOUP_E_DEVICE, "soup_step_batch: the batch's descriptor tables could not be written");
+  }
+  bool graphs = true;
+  for (uint32_t r = 0; r < count; ++r) graphs = graphs && B.hs[r]->use_graphs;
+  const uint32_t P = step_period(B.hs[0]->c);
+  std::vector<uint64_t> start(count);
+  uint32_t done = 0;
+  while (done < nsteps) {
+    const uint32_t chunk = nsteps - done < 1000u ? nsteps - done : 1000u;
+    for (uint32_t r = 0; r < count; ++r) start[r] = B.hs[r]->global_step;
+    for (uint32_t k = 0; k < chunk;) {
+      if (graphs && B.hs[0]->global_step % P == 0 && chunk - k >= P && pk_period(B, P)) {
+        k += P;
+        continue;
+      }
+      pk_step(B, false, false);
+      ++k;
+    }
+    const hipError_t e = hipStreamSynchronize(B.stream);
+    if (e != hipSuccess) {
+      *failed = 0;
+      return fail(B.hs[0], SOUP_E_DEVICE, std::string("soup_step_batch: packed steps: ") + hipGetErrorString(e));
+    }
+    for (uint32_t r = 0; r < count; ++r) {
+      const int rc = chunk_end(B.hs[r], start[r]);
+      if (rc) {
+        *failed = r;
+        return rc;
+      }
+    }
+    done += chunk;
+    // a sweep rebuilt some systems on their own streams, which flips their buffer assignment
+    keys_ok = true;
+    for (uint32_t r = 0; r < count; ++r) keys_ok = keys_ok && B.keys[r] == B.hs[r]->b.pos;
+    if (!keys_ok && pack_tables(B) != hipSuccess) {
+      *failed = 0;
+      return fail(B.hs[0], SOUP_E_DEVICE, "soup_step_batch: the batch's descriptor tables could not be written");
+    }
+  }
+  return 0;
+}
+
+extern "C" {
+
 int soup_step_batch(SoupHandle* const* hs, uint32_t count, uint32_t nsteps, uint32_t* failed) {
   if (!hs || (count && !failed)) return SOUP_E_INVALID;
   for (uint32_t s = 0; s < count; ++s)
     if (!hs[s]) return SOUP_E_INVALID;
+  if (count && nsteps && packable(hs, count)) {
+    std::string why;
+    BatchCtx* B = batch_of(hs, count, &why);
+    if (B) return step_packed(*B, nsteps, failed);
+    *failed = 0;
+    return fail(hs[0], SOUP_E_DEVICE, why);
+  }
   std::vector<uint64_t> start(count);
   std::vector<uint32_t> left(count), period(count);
   uint32_t done = 0;
