@@ -8,10 +8,11 @@
 // the particle index (the reference's array index), inv[id] its current slot. Bond slots and
 // centerLink keep holding particle indices, exactly as in the reference.
 //
-// The Verlet list is stored transposed, vl[s * n + slot], so lane k of a wavefront reads entry s of
-// slot base+k in one coalesced 256-byte transaction. Entries are SLOTS (positions in the same
-// cell-sorted arrays), in the canonical order of soup/wgsl/verlet.wgsl:66-88 (z-major cell walk,
-// ascending particle index inside each cell).
+// The Verlet list is row-major, vl[slot * stride + s], with the stride (vl_stride) the capacity rounded
+// up to 32 entries so every row starts on its own 128-byte line. Entries are SLOTS (positions in the
+// same cell-sorted arrays). A row holds its near part (closer than the near/far split at the rebuild)
+// from the front in the canonical order of soup/wgsl/verlet.wgsl:66-88 (z-major cell walk, ascending
+// particle index inside each cell) and its far part from the back of the capacity in reverse order.
 //
 // The DH list is row-major, es[h * es_cap + s]: row h is the h-th charged-kind particle in ascending
 // particle-index order (electrostatics-long.wgsl:15-29); its entries are the neighbour HEAD indices,
@@ -138,3 +139,7 @@ __device__ __forceinline__ uint32_t xcd_block() {
 }
 
 __device__ __forceinline__ bool nonfinite_bits(float x) { return ((__float_as_uint(x) >> 23) & 0xFFu) == 0xFFu; }
+
+// Stored stride of a Verlet row: the capacity rounded up to 32 entries, so that every row starts on
+// its own 128-byte line (the capacity itself, 1126 at the campaign, keeps the overflow semantics)
+__host__ __device__ __forceinline__ uint32_t vl_stride(const SoupConfig& c) { return (c.list_cap + 31u) & ~31u; }

@@ -507,7 +507,7 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
   wave_lds_sync();
   // ---- dense sweep ----
   const uint32_t cap = c.list_cap;
-  uint32_t* row = b.vl + (size_t)t * cap;
+  uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   const uint64_t below = (1ull << lane) - 1ull;
   const float split = list_split(c);
   const float split2 = split * split;
@@ -957,7 +957,7 @@ __device__ __forceinline__ void force_group_body(SoupConfig c, DevBuf b, int es,
   const uint32_t t = valid ? g : c.n - 1u;
   const PairConst P = make_pair_const(c);
   const uint32_t cap = c.list_cap;
-  const uint32_t* row = b.vl + (size_t)t * cap;
+  const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   const float4 xi = b.pos[t];
   const float qi = b.charge[t];
   const uint32_t cn = b.vnear[t];
@@ -1116,7 +1116,7 @@ __device__ __forceinline__ void bond_propose_body(SoupConfig c, DevBuf b, const 
   uint32_t myJ = SOUP_NONE, myR = 0;  // candidate k lives in lane k
   // every candidate and catalyst lies in the near part of the row, which keeps the walk order
   const uint32_t cnt = b.vnear[t];
-  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   for (uint32_t base = 0; base < cnt && (catNear == 0u || nCand < SOUP_CAND_CAP); base += WAVE) {
     const uint32_t s = base + lane;
     bool isCat = false, isCand = false;
@@ -1761,7 +1761,7 @@ __global__ __launch_bounds__(256) void k_potential(SoupConfig c, DevBuf b, SoupP
   const double cut2 = pb.cutoff * pb.cutoff;
   double nb = 0.0, fene = 0.0, bend = 0.0, teth = 0.0;
   const uint32_t cnt = uni(b.vcount[t]), cn = uni(b.vnear[t]);
-  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   for (uint32_t s = lane; s < cnt; s += WAVE) {
     const uint32_t u = row[row_index(s, cn, c.list_cap)];
     if (b.id[u] <= idi) continue;
@@ -1930,7 +1930,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
 #pragma unroll
   for (uint32_t p = 0; p < ADJ_P; ++p) A.dU[p] = A.dF[p] = 0.0;
   const uint32_t cnt = uni(b.vcount[t]), cn = uni(b.vnear[t]);
-  const uint32_t* row = b.vl + (size_t)t * c.list_cap;
+  const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   for (uint32_t s = lane; s < cnt; s += WAVE) {
     const uint32_t u = row[row_index(s, cn, c.list_cap)];
     const float4 xj = b.pos[u];
@@ -2336,7 +2336,7 @@ __global__ void k_vl_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
   const uint32_t cap = c.list_cap, i = b.id[t], ct = b.cell_of[t];
-  const uint32_t* row = b.vl + (size_t)t * cap;
+  const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   const uint32_t cnt = b.vcount[t], cn = b.vnear[t];
   uint32_t pn = 0, pf = cn;  // next near entry, next far entry (as row_index positions)
   for (uint32_t s = 0; s < cap; ++s) {

@@ -625,7 +625,7 @@ static int upload_state(SoupHandle* h, uint32_t n, const SoupInitState* st) {
 static int check_device_limit(const SoupConfig& c, uint32_t cap, uint32_t nh) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;  // no figure to compare with: hipMalloc decides
-  const size_t list = (size_t)cap * c.list_cap * 4;
+  const size_t list = (size_t)cap * vl_stride(c) * 4;
   const size_t es_rows = (size_t)nh * c.es_cap;
   struct Item {
     const char* what;
@@ -700,7 +700,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.cell_rank, n);
         DA(h, b.cell_tmp, n);
         DA(h, b.perm, n);
-        DA(h, b.vl, (size_t)n * cfg->list_cap);
+        DA(h, b.vl, (size_t)n * vl_stride(*cfg));
         DA(h, b.vcount, n);
         DA(h, b.vnear, n);
         DA(h, b.flags, 10);
