@@ -98,7 +98,12 @@ typedef struct SoupConfig {
   uint32_t rebuild_mode;
   uint32_t drift_check_every;
   float drift_trigger;
-  uint32_t reserved[3];
+  /* The list build stages each cell's (2R + 1)^3-cell neighbourhood in LDS when it holds at most
+   * this many beads (0: the library's limit, 4608 less the 256 entries of padding the sweep reads past them; larger values are capped at it) and gathers the
+   * candidates from the particle arrays otherwise. Both ways give the same rows; a small value
+   * forces the gather way (a launch-shape knob with no counterpart in the reference). */
+  uint32_t list_stage;
+  uint32_t reserved[2];
 } SoupConfig;
 
 enum { SOUP_REBUILD_CADENCE = 0, SOUP_REBUILD_DRIFT = 1 };

@@ -424,26 +424,81 @@ __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t c
 
 // ================================================================================================
 // Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92), one wavefront per particle.
-// Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Slots
-// are cell-major with x fastest, so the kept cells of one (dz, dy) row are one contiguous slot range
-// (two when the row wraps in x), visited in canonical order. Lane r first sets up row r (cells lying
-// entirely beyond listRange + 0.01 sigma of the particle are dropped: they hold no entry, so the list
-// is unchanged) and a scan over the lanes lays the kept ranges end to end; the wavefront then sweeps
-// that one dense candidate range, VL_BATCH chunks of 64 per round of loads (each lane carries its
-// row from chunk to chunk, since its index only grows), and ballots compact the
-// hits in order: the near ones (list_split) from the front of the row, the far ones from its back.
-// Needs (2R + 1)^2 <= 64 rows.
+// Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Lane r
+// first sets up row r = (dz, dy): cells lying entirely beyond listRange + 0.01 sigma of the particle
+// are dropped (they hold no entry, so the list is unchanged), which leaves one interval of dx around
+// dx = 0; a scan over the lanes lays the kept ranges end to end, and the wavefront sweeps that one
+// dense candidate range, VL_BATCH chunks of 64 at a time (each lane carries its row from chunk to
+// chunk, since its index only grows). Ballots compact the hits in order: the near ones (list_split)
+// from the front of the row, the far ones from its back. Needs (2R + 1)^2 <= 64 rows.
+//
+// Two ways to reach the candidates:
+// - k_verlet_cells (the rebuild): one workgroup per cell. It first stages the (2R + 1)^3 cells around
+//   its cell in LDS, in canonical order, each position with its image shift already added and its
+//   slot in w; then each wavefront builds the rows of the cell's particles one after the other,
+//   reading candidates from LDS by the flat index. A cell whose stencil holds more than VL_STAGE
+//   beads (or the configuration's smaller list_stage) builds its rows from the particle arrays
+//   instead, as below.
+// - verlet_row_global: candidates gathered from the particle arrays. Slots are cell-major with x
+//   fastest, so the kept cells of a row are one contiguous slot range (two when the row wraps in x).
 // ================================================================================================
 #ifndef VL_BATCH
 #define VL_BATCH 4u  // Verlet-build candidate chunks of 64 whose loads are issued together
 #endif
-__device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const uint32_t blk) {
-  __shared__ uint32_t s_end[256 / WAVE][WAVE];  // inclusive prefix of kept row lengths
-  __shared__ uint4 s_seg[256 / WAVE][WAVE];     // first slot of range a, its length, first slot of range b
-  __shared__ float4 s_sh[256 / WAVE][WAVE];     // image shifts: x of range a, x of range b, y, z
-  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
-  const uint32_t t = uni(blk * (blockDim.x / WAVE) + w);
-  if (t >= c.n) return;
+#ifndef VL_CELLS
+#define VL_CELLS 1u  // 1: the rebuild runs k_verlet_cells, 0: k_verlet_build (one wavefront per particle)
+#endif
+#ifndef VL_STAGE
+#define VL_STAGE 4608u  // stencil beads one workgroup of k_verlet_cells stages in LDS (16 B each)
+#endif
+#define VL_WAVES 16u    // wavefronts per workgroup of k_verlet_cells (two workgroups fill a CU)
+#define VL_PAD (VL_BATCH * WAVE)  // staged padding the sweep may read past the stencil
+static_assert(VL_STAGE >= 2u * VL_WAVES * WAVE + VL_WAVES * WAVE / 4u, "the gather path's tables alias the staging");
+
+// Margin of the cheap list test. The squared distance taken through the stencil's known image shift
+// differs from the reference's exact f32 test (minimum image by rint, then the length) by at most
+// about 6 sqrt(3) L u (r + 1) + 8 u r^2 (sigma^2; u = 2^-24, L the longest box edge, r the range):
+// 2.7e-4 at L = 76. Candidates nearer than r - margin or farther than r + margin are decided by the
+// cheap test, the band between by the exact one; the margin (in distance) is at least 10x that bound
+// for any box, and it also covers the f32 rounding of the cell gaps the row set-up culls with.
+__device__ __forceinline__ float list_margin(const SoupConfig& c) {
+  return fmaxf(0.001f, 4e-6f * fmaxf(c.box[0], fmaxf(c.box[1], c.box[2])));
+}
+
+// the row's counts; past the capacity (an overflow, reported) the two parts overlap
+__device__ __forceinline__ void verlet_finish(const DevBuf& b, uint32_t t, uint32_t cn, uint32_t cf, uint32_t cap) {
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    const uint32_t count = cn + cf;
+    b.vcount[t] = count < cap ? count : cap;
+    b.vnear[t] = cn < cap ? cn : cap;
+    if (count > cap) b.flags[0] = 1u;
+  }
+}
+
+// the hits of one chunk of 64 candidates, compacted in lane order into the row
+__device__ __forceinline__ void verlet_emit(uint32_t* row, uint32_t cap, uint64_t below, bool hit, bool near, uint32_t kk,
+                                            uint32_t& cn, uint32_t& cf) {
+  const uint64_t mn = __ballot(hit && near), mf = __ballot(hit && !near);
+  if (hit) {
+    // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row
+    if (near) {
+      const uint32_t p = cn + (uint32_t)__popcll(mn & below);
+      if (p < cap) row[p] = kk;
+    } else {
+      const uint32_t p = cf + (uint32_t)__popcll(mf & below);
+      if (p < cap) row[cap - 1u - p] = kk;
+    }
+  }
+  cn += (uint32_t)__popcll(mn);
+  cf += (uint32_t)__popcll(mf);
+}
+
+// particle t's row from candidates gathered in the particle arrays; s_end, s_seg and s_sh are this
+// wavefront's 64-entry LDS tables
+__device__ __forceinline__ void verlet_row_global(const SoupConfig& c, const DevBuf& b, const uint32_t t, uint32_t* s_end,
+                                                  uint4* s_seg, float4* s_sh) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  wave_lds_sync();  // the tables may still hold the wavefront's previous row
   const float4 xv = b.pos[t];
   const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
   const uint32_t cell = uni(b.cell_of[t]);
@@ -453,8 +508,9 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
   const float lo0 = xi.x - (float)cx * w0, lo1 = xi.y - (float)cy * w1, lo2 = xi.z - (float)cz * w2;
   const float hi0 = w0 - lo0, hi1 = w1 - lo1, hi2 = w2 - lo2;
   const float lr = c.list_range;
-  const float cull = (lr + 0.01f) * (lr + 0.01f);
-  const float sure = (lr - 0.01f) * (lr - 0.01f);  // inside this the exact f32 test cannot fail
+  const float mg = list_margin(c);
+  const float cull = (lr + mg) * (lr + mg);
+  const float sure = (lr - mg) * (lr - mg);  // inside this the exact f32 test cannot fail
   const int R = (int)c.walk_radius, side = 2 * R + 1;
   const uint32_t nrows = (uint32_t)(side * side);
   // ---- row set-up, lane r = row (dz, dy) in canonical order ----
@@ -500,9 +556,9 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
     const uint32_t y = (uint32_t)__shfl_up((int)incl, o, WAVE);
     if (lane >= (uint32_t)o) incl += y;
   }
-  s_end[w][lane] = incl;
-  s_seg[w][lane] = make_uint4(k0a, lena, k0b, 0u);
-  s_sh[w][lane] = make_float4(sxa, sxb, sy, sz);
+  s_end[lane] = incl;
+  s_seg[lane] = make_uint4(k0a, lena, k0b, 0u);
+  s_sh[lane] = make_float4(sxa, sxb, sy, sz);
   const uint32_t total = uni((uint32_t)__shfl((int)incl, WAVE - 1, WAVE));
   wave_lds_sync();
   // ---- dense sweep ----
@@ -514,7 +570,7 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
   uint32_t cn = 0, cf = 0;  // near and far hits so far
   // each lane's row and that row's start: a lane's index only grows (by 64 a chunk), so the row is
   // carried from chunk to chunk and advanced past the rows that end at or before the index
-  uint32_t r = 0, rstart = 0, rend = s_end[w][0];
+  uint32_t r = 0, rstart = 0, rend = s_end[0];
   for (uint32_t base = 0; base < total; base += VL_BATCH * WAVE) {
     uint32_t kk[VL_BATCH];
     float3 sh[VL_BATCH];
@@ -525,13 +581,13 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
       kk[k] = t;
       sh[k] = make_float3(0.0f, 0.0f, 0.0f);
       if (f < total) {
-        while (rend <= f) {  // f < total = s_end[w][nrows - 1], so r stays below nrows
+        while (rend <= f) {  // f < total = s_end[nrows - 1], so r stays below nrows
           rstart = rend;
-          rend = s_end[w][++r];
+          rend = s_end[++r];
         }
         const uint32_t loc = f - rstart;
-        const uint4 sg = s_seg[w][r];
-        const float4 sv = s_sh[w][r];
+        const uint4 sg = s_seg[r];
+        const float4 sv = s_sh[r];
         const bool in_a = loc < sg.y;
         kk[k] = in_a ? sg.x + loc : sg.z + (loc - sg.y);
         sh[k] = make_float3(in_a ? sv.x : sv.y, sv.z, sv.w);
@@ -549,29 +605,244 @@ __device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const 
         if (!hit && a2 <= cull) hit = dist_f(c, xi, xj[k]) < lr;
         near = a2 < split2;
       }
-      const uint64_t mn = __ballot(hit && near), mf = __ballot(hit && !near);
-      if (hit) {
-        // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row
-        if (near) {
-          const uint32_t p = cn + (uint32_t)__popcll(mn & below);
-          if (p < cap) row[p] = kk[k];
-        } else {
-          const uint32_t p = cf + (uint32_t)__popcll(mf & below);
-          if (p < cap) row[cap - 1u - p] = kk[k];
-        }
+      verlet_emit(row, cap, below, hit, near, kk[k], cn, cf);
+    }
+  }
+  verlet_finish(b, t, cn, cf, cap);
+}
+
+__device__ __forceinline__ void verlet_build_body(SoupConfig c, DevBuf b, const uint32_t blk) {
+  __shared__ uint32_t s_end[256 / WAVE][WAVE];  // inclusive prefix of kept row lengths
+  __shared__ uint4 s_seg[256 / WAVE][WAVE];     // first slot of range a, its length, first slot of range b
+  __shared__ float4 s_sh[256 / WAVE][WAVE];     // image shifts: x of range a, x of range b, y, z
+  const uint32_t w = uni(threadIdx.x / WAVE);
+  const uint32_t t = uni(blk * (blockDim.x / WAVE) + w);
+  if (t >= c.n) return;
+  verlet_row_global(c, b, t, s_end[w], s_seg[w], s_sh[w]);
+}
+__global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) { verlet_build_body(c, b, xcd_block()); }
+
+// what the rows of one cell's particles share: cell widths, thresholds, and per lane r < (2R + 1)^2
+// the row's gap terms (gap along z = max(az + sz * lo_z, 0) with lo_z the particle's offset in its
+// cell, likewise y)
+struct VlShared {
+  float w0, lo_base0, lo_base1, lo_base2;  // cell width in x; the cell's lower corner
+  float cull, sure, split2, lr;
+  float az, sz, ay, sy;  // this lane's row terms
+  uint32_t k_row;        // this lane's first stencil cell (dx = -R)
+  bool row_ok;           // lane < (2R + 1)^2
+};
+
+__device__ __forceinline__ VlShared verlet_shared(const SoupConfig& c, int cx, int cy, int cz) {
+  VlShared v;
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const float w0 = c.box[0] / (float)c.dims[0], w1 = c.box[1] / (float)c.dims[1], w2 = c.box[2] / (float)c.dims[2];
+  v.w0 = w0;
+  v.lo_base0 = (float)cx * w0;
+  v.lo_base1 = (float)cy * w1;
+  v.lo_base2 = (float)cz * w2;
+  v.lr = c.list_range;
+  const float mg = list_margin(c);
+  v.cull = (v.lr + mg) * (v.lr + mg);
+  v.sure = (v.lr - mg) * (v.lr - mg);  // inside this the exact f32 test cannot fail
+  const float split = list_split(c);
+  v.split2 = split * split;
+  const int R = (int)c.walk_radius, side = 2 * R + 1;
+  v.row_ok = lane < (uint32_t)(side * side);
+  const int dz = (int)lane / side - R, dy = (int)lane % side - R;
+  // gap to the row's cells: dz > 0: dz w - lo, dz < 0: lo - (dz + 1) w... written as a + s lo
+  v.az = dz > 0 ? (float)dz * w2 : (dz < 0 ? (float)(-dz - 1) * w2 : 0.0f);
+  v.sz = dz > 0 ? -1.0f : (dz < 0 ? 1.0f : 0.0f);
+  v.ay = dy > 0 ? (float)dy * w1 : (dy < 0 ? (float)(-dy - 1) * w1 : 0.0f);
+  v.sy = dy > 0 ? -1.0f : (dy < 0 ? 1.0f : 0.0f);
+  v.k_row = lane * (uint32_t)side;
+  return v;
+}
+
+// particle t's row from the stencil staged in LDS (s_pos, with s_cpre the exclusive prefix of the
+// stencil cells' counts in canonical order).
+// Flat candidate f of the row set lies in row r when end[r - 1] <= f < end[r] and sits at staged
+// index f + beg[r] - end[r - 1]. Lane r holds end[r] and beg[r]; the wavefront walks the row
+// boundaries that fall inside each chunk of 64 with uniform reads of those lanes, so a lane's staged
+// index needs no LDS lookup. The cell culling only has to be conservative (a dropped cell lies
+// beyond listRange + the margin), so its gaps are not computed in any particular rounding.
+__device__ __forceinline__ void verlet_row_staged(const SoupConfig& c, const DevBuf& b, const uint32_t t, const VlShared& v,
+                                                  const float4* s_pos, const uint32_t* s_cpre) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const float4 xv = b.pos[t];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
+  const float lo0 = xi.x - v.lo_base0, lo1 = xi.y - v.lo_base1, lo2 = xi.z - v.lo_base2;
+  const float hi0 = v.w0 - lo0;
+  const float cull = v.cull, sure = v.sure, lr = v.lr;
+  const int R = (int)c.walk_radius;
+  // ---- row set-up, lane r = row (dz, dy) in canonical order: its kept cells dx = dxl..dxh are the
+  // stencil cells r * side + R + dxl .. r * side + R + dxh, one contiguous range of the staging ----
+  uint32_t beg = 0, len = 0;
+  const float gz = fmaxf(v.az + v.sz * lo2, 0.0f), gy = fmaxf(v.ay + v.sy * lo1, 0.0f);
+  const float gyz = gy * gy + gz * gz;
+  if (v.row_ok && gyz <= cull) {
+    int dxl = 0, dxh = 0;
+    for (int dx = 1; dx <= R; ++dx) {
+      const float gh = fmaxf((float)(dx - 1) * v.w0 + hi0, 0.0f), gl = fmaxf((float)(dx - 1) * v.w0 + lo0, 0.0f);
+      if (gh * gh + gyz <= cull) dxh = dx;
+      if (gl * gl + gyz <= cull) dxl = -dx;
+    }
+    const uint32_t k0 = v.k_row + (uint32_t)(R + dxl);
+    beg = s_cpre[k0];
+    len = s_cpre[k0 + (uint32_t)(dxh - dxl) + 1u] - beg;
+  }
+  uint32_t end = len;  // inclusive prefix over the lanes: end[r]
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)end, o, WAVE);
+    if (lane >= (uint32_t)o) end += y;
+  }
+  const uint32_t total = uni((uint32_t)__builtin_amdgcn_readlane((int)end, WAVE - 1));
+  // ---- dense sweep ----
+  const uint32_t cap = c.list_cap;
+  uint32_t* row = b.vl + (size_t)t * vl_stride(c);
+  const float split2 = v.split2;
+  uint32_t cn = 0, cf = 0;  // near and far hits so far
+  // uniform walk state: the row r of the next chunk's first candidate, that row's end and the
+  // offset (staged index - flat index) of its candidates
+  uint32_t r = 0, rend = uni((uint32_t)__builtin_amdgcn_readlane((int)end, 0));
+  uint32_t off = uni((uint32_t)__builtin_amdgcn_readlane((int)beg, 0));
+  for (uint32_t base = 0; base < total; base += VL_BATCH * WAVE) {
+    float4 xj[VL_BATCH];
+#pragma unroll
+    for (uint32_t k = 0; k < VL_BATCH; ++k) {
+      const uint32_t cb = base + k * WAVE, f = cb + lane;
+      uint32_t o = off;
+      // boundaries inside this chunk: lanes at or past end[r] belong to a later row (empty rows
+      // have end[r] = end[r - 1] and are passed at once); f < total keeps r below nrows
+      while (rend < cb + WAVE && rend < total) {
+        const uint32_t pb = rend - cb;
+        ++r;
+        off = uni((uint32_t)__builtin_amdgcn_readlane((int)beg, (int)r)) - rend;
+        if (lane >= pb) o = off;
+        rend = uni((uint32_t)__builtin_amdgcn_readlane((int)end, (int)r));
       }
+      // lanes past the row set read the culled cells after it or the far-away padding that follows
+      // the staging: no hit either way
+      xj[k] = s_pos[f + o];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < VL_BATCH; ++k) {
+      const uint32_t kk = __float_as_uint(xj[k].w);
+      // cheap test on the shifted position, the reference's exact one on the stored position in the
+      // narrow band between sure and cull
+      const float ax = xj[k].x - xi.x, ay = xj[k].y - xi.y, az = xj[k].z - xi.z;
+      const float a2 = ax * ax + ay * ay + az * az;
+      bool hit = a2 <= sure;
+      if (__builtin_expect(!hit && a2 <= cull, 0)) hit = dist_f(c, xi, b.pos[kk]) < lr;
+      hit = hit && kk != t;
+      const bool near = a2 < split2;
+      // ballots compact the hits in lane order: near ones from the front, far ones from the back
+      const uint64_t mh = __builtin_amdgcn_ballot_w64(hit), mn = __builtin_amdgcn_ballot_w64(hit && near);
+      const uint64_t mf = mh & ~mn;
+      const uint32_t pn = __builtin_amdgcn_mbcnt_hi((uint32_t)(mn >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mn, cn));
+      const uint32_t pf = __builtin_amdgcn_mbcnt_hi((uint32_t)(mf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mf, cf));
+      // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row
+      const uint32_t pp = near ? pn : pf;
+      if (hit && pp < cap) row[near ? pp : cap - 1u - pp] = kk;
       cn += (uint32_t)__popcll(mn);
       cf += (uint32_t)__popcll(mf);
     }
   }
-  if (lane == 0) {
-    const uint32_t count = cn + cf;
-    b.vcount[t] = count < cap ? count : cap;
-    b.vnear[t] = cn < cap ? cn : cap;
-    if (count > cap) b.flags[0] = 1u;
+  verlet_finish(b, t, cn, cf, cap);
+}
+
+// one workgroup per cell (logical block = cell index)
+__device__ __forceinline__ void verlet_cell_body(SoupConfig c, DevBuf b, const uint32_t cellid) {
+  __shared__ float4 s_pos[VL_STAGE];  // stencil beads: position + image shift, w = slot bits
+  __shared__ uint32_t s_cpre[344];    // exclusive prefix of the (2R + 1)^3 <= 343 stencil cells' counts
+  __shared__ uint2 s_cinfo[343];      // per stencil cell: first slot, image-shift code
+  __shared__ uint32_t s_wsum[VL_WAVES];
+  const uint32_t ncells = c.dims[0] * c.dims[1] * c.dims[2];
+  if (cellid >= ncells) return;
+  const uint32_t first = uni(b.cell_start[cellid]), np = uni(b.cell_start[cellid + 1u] - first);
+  if (np == 0u) return;  // the whole workgroup leaves together
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = uni(threadIdx.x / WAVE);
+  const int cx = (int)(cellid % c.dims[0]), cy = (int)((cellid / c.dims[0]) % c.dims[1]), cz = (int)(cellid / (c.dims[0] * c.dims[1]));
+  const int R = (int)c.walk_radius, side = 2 * R + 1;
+  const uint32_t nst = (uint32_t)(side * side * side);
+  // ---- stencil cell k = threadIdx.x in canonical order (dz = k / side^2 - R, dy = (k / side) % side
+  // - R, dx = k % side - R): its slot range and image shift (code bits 2a / 2a + 1: -L / +L on axis
+  // a), then the exclusive prefix of the counts (nst <= 343, fewer than the workgroup's threads) ----
+  uint32_t cnt = 0;
+  if (threadIdx.x < nst) {
+    const uint32_t k = threadIdx.x;
+    int x = cx + (int)(k % (uint32_t)side) - R, y = cy + (int)((k / (uint32_t)side) % (uint32_t)side) - R,
+        z = cz + (int)(k / (uint32_t)(side * side)) - R;
+    uint32_t code = 0;
+    if (x < 0) { x += (int)c.dims[0]; code |= 1u; } else if (x >= (int)c.dims[0]) { x -= (int)c.dims[0]; code |= 2u; }
+    if (y < 0) { y += (int)c.dims[1]; code |= 4u; } else if (y >= (int)c.dims[1]) { y -= (int)c.dims[1]; code |= 8u; }
+    if (z < 0) { z += (int)c.dims[2]; code |= 16u; } else if (z >= (int)c.dims[2]) { z -= (int)c.dims[2]; code |= 32u; }
+    const uint32_t cell = (uint32_t)x + c.dims[0] * ((uint32_t)y + c.dims[1] * (uint32_t)z);
+    const uint32_t cs = b.cell_start[cell];
+    cnt = b.cell_start[cell + 1u] - cs;
+    s_cinfo[k] = make_uint2(cs, code);
+  }
+  uint32_t incl = cnt;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, WAVE);
+    if (lane >= (uint32_t)o) incl += y;
+  }
+  if (lane == WAVE - 1) s_wsum[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t v = 0; v < w; ++v) off += s_wsum[v];
+  if (threadIdx.x < nst) s_cpre[threadIdx.x] = off + incl - cnt;
+  if (threadIdx.x == nst - 1u) s_cpre[nst] = off + incl;
+  __syncthreads();
+  const uint32_t total = s_cpre[nst];
+  const uint32_t stage_cap = (c.list_stage != 0u && c.list_stage < VL_STAGE) ? c.list_stage : VL_STAGE;
+  if (total + VL_PAD <= stage_cap) {
+    // far-away padding after the stencil: the sweep reads up to VL_PAD entries past it
+    for (uint32_t e = threadIdx.x; e < VL_PAD; e += VL_WAVES * WAVE)
+      s_pos[total + e] = make_float4(3.0e37f, 3.0e37f, 3.0e37f, 0.0f);
+    // ---- stage: wavefront w copies stencil cells w, w + VL_WAVES, ..., eight cells' loads at a time ----
+    for (uint32_t k0 = w; k0 < nst; k0 += 8u * VL_WAVES) {
+      float4 p[8];
+      uint32_t cs[8], code[8], n[8], dst[8];
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) {
+        const uint32_t k = k0 + j * VL_WAVES;
+        n[j] = cs[j] = code[j] = dst[j] = 0u;
+        if (k < nst) {
+          const uint2 ci = s_cinfo[k];
+          cs[j] = uni(ci.x);
+          code[j] = uni(ci.y);
+          dst[j] = uni(s_cpre[k]);
+          n[j] = uni(s_cpre[k + 1u]) - dst[j];
+        }
+        if (lane < n[j]) p[j] = b.pos[cs[j] + lane];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 8u; ++j) {
+        const float sx = (code[j] & 1u) ? -c.box[0] : ((code[j] & 2u) ? c.box[0] : 0.0f);
+        const float sy = (code[j] & 4u) ? -c.box[1] : ((code[j] & 8u) ? c.box[1] : 0.0f);
+        const float sz = (code[j] & 16u) ? -c.box[2] : ((code[j] & 32u) ? c.box[2] : 0.0f);
+        if (lane < n[j]) s_pos[dst[j] + lane] = make_float4(p[j].x + sx, p[j].y + sy, p[j].z + sz, __uint_as_float(cs[j] + lane));
+        for (uint32_t e = lane + WAVE; e < n[j]; e += WAVE) {  // cells of more than 64 beads
+          const float4 q = b.pos[cs[j] + e];
+          s_pos[dst[j] + e] = make_float4(q.x + sx, q.y + sy, q.z + sz, __uint_as_float(cs[j] + e));
+        }
+      }
+    }
+    __syncthreads();
+    const VlShared vs = verlet_shared(c, cx, cy, cz);
+    for (uint32_t i = w; i < np; i += VL_WAVES) verlet_row_staged(c, b, first + i, vs, s_pos, s_cpre);
+  } else {
+    // stencil too dense to stage: gather from the particle arrays (the tables alias the staging)
+    uint4* seg = reinterpret_cast<uint4*>(s_pos) + w * WAVE;
+    float4* sh = s_pos + (VL_WAVES + w) * WAVE;
+    uint32_t* ends = reinterpret_cast<uint32_t*>(s_pos + 2u * VL_WAVES * WAVE) + w * WAVE;
+    for (uint32_t i = w; i < np; i += VL_WAVES) verlet_row_global(c, b, first + i, ends, seg, sh);
   }
 }
-__global__ __launch_bounds__(256) void k_verlet_build(SoupConfig c, DevBuf b) { verlet_build_body(c, b, xcd_block()); }
+__global__ __launch_bounds__(VL_WAVES * WAVE, 2 * VL_WAVES / 4) void k_verlet_cells(SoupConfig c, DevBuf b) {
+  verlet_cell_body(c, b, xcd_block());
+}
 
 // ================================================================================================
 // DH head list: soup_build_es_list_main (electrostatics-long.wgsl:31-56), one wavefront per head,

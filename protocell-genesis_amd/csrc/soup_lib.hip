@@ -268,7 +268,12 @@ static void ph_rebuild(SoupHandle* h) {
     prof_end(h, SOUP_PROF_ES_BUILD, e1);
   }
   hipEvent_t e2 = prof_begin(h, SOUP_PROF_VERLET_BUILD);
-  hipLaunchKernelGGL(k_verlet_build, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  if (VL_CELLS) {
+    const uint64_t ncells = (uint64_t)h->c.dims[0] * h->c.dims[1] * h->c.dims[2];
+    hipLaunchKernelGGL(k_verlet_cells, dim3(xcd_blocks(ncells, 1)), dim3(VL_WAVES * WAVE), 0, h->stream, h->c, h->b);
+  } else {
+    hipLaunchKernelGGL(k_verlet_build, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
+  }
   prof_end(h, SOUP_PROF_VERLET_BUILD, e2);
   fill(h, h->b.flags + 2, 0u, 1);
 }

@@ -51,7 +51,7 @@ class SoupConfig(C.Structure):
         ("catalyst_kind", C.c_float), ("head_terminal_only", C.c_float), ("head_chain_cap", C.c_float),
         ("max_hold_steps", C.c_float), ("bond_interval_f", C.c_float), ("bond_interval", C.c_uint32),
         ("capacity", C.c_uint32), ("rebuild_mode", C.c_uint32), ("drift_check_every", C.c_uint32),
-        ("drift_trigger", C.c_float), ("reserved", C.c_uint32 * 3),
+        ("drift_trigger", C.c_float), ("list_stage", C.c_uint32), ("reserved", C.c_uint32 * 2),
     ]
 
 

@@ -251,6 +251,10 @@ def build_config(R: Resolved, opts: Dict[str, Any]) -> native.SoupConfig:
     c.max_hold_steps = soup["adsorption"]["maxHoldSteps"]
     c.bond_interval_f = interval
     c.bond_interval = interval
+    stage = int((opts.get("verletOverride") or {}).get("stageCapacity", 0))
+    if stage < 0:
+        raise ValueError(f"verletOverride.stageCapacity={stage} must be >= 0 (0: the library's limit)")
+    c.list_stage = stage
     if R.rebuild["mode"] == "drift":
         c.rebuild_mode = native.REBUILD_DRIFT
         c.drift_check_every = R.rebuild["checkEvery"]

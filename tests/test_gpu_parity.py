@@ -35,6 +35,53 @@ def test_priming_lists_match_oracle(broth):
     assert np.array_equal(s.verletList(), o.verlet_rows())
 
 
+def _stencil_totals(cell_start, dims, R):
+    """Beads in each cell's (2R + 1)^3 neighbourhood (periodic)."""
+    cnt = np.diff(np.asarray(cell_start, dtype=np.int64)).reshape(dims[2], dims[1], dims[0])
+    tot = np.zeros_like(cnt)
+    for dz in range(-R, R + 1):
+        for dy in range(-R, R + 1):
+            for dx in range(-R, R + 1):
+                tot += np.roll(cnt, (dz, dy, dx), axis=(0, 1, 2))
+    return tot.ravel()
+
+
+def test_staged_and_gathered_list_builds_match_oracle():
+    """The rebuild stages each cell's neighbourhood in LDS unless it holds more beads than
+    verletOverride.stageCapacity, and then gathers that cell's candidates from the particle arrays.
+    A capacity at the median neighbourhood sends about half the cells each way, 64 sends every cell
+    to the gather path; every row stays bit-exact against the oracle in both cases and equal to the
+    all-staged build's, at creation and after 60 steps."""
+    base = charged(box=20.0, seed=6, pH=6.0)
+    R = ps.resolve(base)
+    ref = ps.createSoup(base)
+    try:
+        dims, rad = list(R.cfg.dims), int(R.cfg.walk_radius)
+        tot = _stencil_totals(ref.readDEBUG("CELL_START"), dims, rad)
+        mid = int(np.median(tot))
+        assert 64 < tot.min() and (tot > mid).any() and (tot <= mid).any()
+        for stage in (mid, 64):
+            s = ps.createSoup(dict(base, verletOverride={"stageCapacity": stage}))
+            try:
+                assert s._R.cfg.list_stage == stage
+                for k in range(2):
+                    if k:
+                        for x in (ref, s):
+                            x.step(60)
+                            x.forces()  # rebuild from the current positions
+                    o = oracle_from_system(s, R)
+                    o.rebuild()
+                    assert np.array_equal(s.verletCount(), o.vcount), (stage, k)
+                    assert np.array_equal(s.verletList(), o.verlet_rows()), (stage, k)
+                    assert np.array_equal(s.verletList(), ref.verletList()), (stage, k)
+            finally:
+                s.dispose()
+            ref.dispose()
+            ref = ps.createSoup(base)
+    finally:
+        ref.dispose()
+
+
 def test_forces_match_fp64(broth):
     R, s = broth
     o = oracle_from_system(s, R)
