@@ -336,6 +336,48 @@ int soup_potential(SoupHandle* h, const SoupPotentialBasis* basis, double out[5]
 int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* basis, const double* w, double* grad, double* hw,
                         double* dU_dtheta, double* dFw_dtheta);
 
+/* soup_energy_adjoint with a diagonal strain e = strain[0..2] added to the direction: every pair term
+ * (list pairs, bonds, 1-3 bends, tethers, DH pairs) takes dw = w_i - w_j + diag(e) r_ij, r_ij the
+ * minimum-image displacement x_i - x_j (w may be NULL: the strain alone). W_e receives the strain
+ * virial sum over pair terms of (diag(e) r_ij) . f_ij, f_ij the term's force on i; hw then holds
+ * H w + H_e, and dW_e/dx = -diag(e) grad - H_e, dW_e/dtheta is the strain's part of dFw_dtheta.
+ * With e = (-1/2, -1/2, 1) / (Lx Ly), W_e is the configurational lateral tension of a membrane
+ * normal to z, (W_zz - (W_xx + W_yy) / 2) / A. No counterpart in the reference (its area move
+ * samples the area instead). */
+int soup_energy_adjoint_strain(SoupHandle* h, const SoupPotentialBasis* basis, const double strain[3], const double* w,
+                               double* grad, double* hw, double* dU_dtheta, double* dFw_dtheta, double* W_e);
+
+/* Trajectory-segment adjoint (the sensitivity of a trajectory observable to the parameters, with
+ * device checkpoints and a forward recompute per stretch). The system advances `steps` steps exactly
+ * as soup_step(h, steps) would; a full checkpoint of the state is kept on the device before every
+ * stretch of checkpoint_every steps. The reverse sweep then recomputes each stretch, last first, on
+ * a replay system from its checkpoint (bit for bit, recording the stretch's states on the device)
+ * and runs the pathwise adjoint of the velocity-Verlet Langevin step (soup/wgsl/step.wgsl:96-142;
+ * the noise does not depend on the state) backwards through it: one force vector-Jacobian product
+ * (k_energy_adjoint) per force evaluation, each at the bond slots, centre links and charges that
+ * force was computed with. Bond and charge Monte Carlo moves are not differentiated.
+ * objective SOUP_OBJ_END_ENERGY: J = soupPotential at the end state;
+ * SOUP_OBJ_MEAN_STRAIN_VIRIAL: J = scale / steps * sum over the states after each step of W_e
+ * (soup_energy_adjoint_strain's virial of the state as the step leaves it).
+ * Out: J, dJ_dtheta[SOUP_ADJ_PARAMS] (theta as soup_energy_adjoint's), and (nullable) dJ_dx0 and
+ * dJ_dv0 [3 N], the gradients with respect to the start positions and velocities (particle order).
+ * Needs the cadence rebuild schedule, a start step and checkpoint_every that are multiples of
+ * rebuildEvery, a start that does not follow a bond Monte Carlo step, and no frozen particles.
+ * Device memory: a checkpoint (about 100 bytes per particle) per stretch and a record of
+ * checkpoint_every + 1 states (40 bytes per particle each), freed on return. No counterpart in the
+ * reference. */
+enum { SOUP_OBJ_END_ENERGY = 0, SOUP_OBJ_MEAN_STRAIN_VIRIAL = 1 };
+typedef struct SoupSegmentSpec {
+  uint32_t steps;
+  uint32_t checkpoint_every;
+  uint32_t objective;
+  uint32_t reserved;
+  double strain[3];
+  double scale;
+} SoupSegmentSpec;
+int soup_segment_adjoint(SoupHandle* h, const SoupPotentialBasis* basis, const SoupSegmentSpec* spec, double* J,
+                         double* dJ_dtheta, double* dJ_dx0, double* dJ_dv0);
+
 /* soupPotential (soup/src/soup-potential.ts:145-301) of an arbitrary configuration of this system:
  * positions (x, y, z, kind per particle, particle order, host memory) in the periodic box `box`,
  * with the system's current bond slots, centre links and charges. Replaces the CPU energy of the area

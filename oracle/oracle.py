@@ -206,6 +206,94 @@ class OracleState:
     def verlet_rows(self) -> np.ndarray:
         return self.vl.reshape(self.n, -1)
 
+    def load(self, positions, bond_slots, center_link, charges):
+        """Replaces positions and topology in place (particle order) and rebuilds the lists."""
+        self.pos[:] = np.asarray(positions, dtype=np.float32).reshape(-1)
+        self.bond_slots[:] = np.asarray(bond_slots, dtype=np.uint32).reshape(-1)
+        self.center_link[:] = np.asarray(center_link, dtype=np.uint32).reshape(-1)
+        self.charge[:] = np.asarray(charges, dtype=np.float32).reshape(-1)
+        self.rebuild()
+
+    def energy_adjoint(self, basis, box, q_head: float, w=None, strain=None) -> dict:
+        """FP64 gradient, Hessian-vector product (direction w plus diag(strain) r per pair term) and
+        parameter derivatives of soupPotential over the state's current lists (soup_oracle.c
+        or_energy_adjoint): {grad, hw, dU, dFw, We}."""
+        n = self.n
+        g = np.zeros(3 * n)
+        hw = np.zeros(3 * n)
+        dU = np.zeros(14)
+        dF = np.zeros(14)
+        we = C.c_double(0.0)
+        bx = np.ascontiguousarray(box, dtype=np.float64)
+        wa = None if w is None else np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        st = None if strain is None else np.ascontiguousarray(strain, dtype=np.float64)
+        lib().or_energy_adjoint(self._s, C.c_uint32(n), C.c_uint32(int(self.cfg.list_cap)), C.c_uint32(int(self.cfg.es_cap)),
+                                C.byref(basis), bx.ctypes.data_as(C.c_void_p), C.c_double(q_head),
+                                None if wa is None else wa.ctypes.data_as(C.c_void_p),
+                                None if st is None else st.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p),
+                                hw.ctypes.data_as(C.c_void_p), dU.ctypes.data_as(C.c_void_p), dF.ctypes.data_as(C.c_void_p),
+                                C.byref(we))
+        return {"grad": g.reshape(n, 3), "hw": hw.reshape(n, 3), "dU": dU, "dFw": dF, "We": we.value}
+
+
+def segment_gradient_f64(state: "OracleState", basis, box, q_head: float, dt: float, gamma: float, start, records, swept,
+                         bond_interval: int, start_step: int, objective: str = "energy", strain=None, scale: float = 1.0):
+    """CPU restatement, in FP64, of the product's trajectory-segment adjoint (soup_segment_adjoint) over a
+    recorded trajectory: ``start`` = (positions, bond slots, centre links, charges) before the first
+    step, ``records[s]`` the same after step s (charges after a sweep where swept[s]), K = len(records).
+    The reverse sweep through the velocity-Verlet Langevin step takes one or_energy_adjoint per force
+    evaluation; objective "energy" is U of the last record, "virial" the mean over records of
+    scale * W_e(strain). Returns {J, dJ_dtheta, dJ_dx0, dJ_dv0}. Test infrastructure: the check of the
+    product's gradient, not a restatement of a reference function."""
+    K = len(records)
+    n = state.n
+    a = 1.0 - gamma * dt
+    st0 = np.zeros(3)
+    obj = None if objective == "energy" else np.asarray(strain, dtype=np.float64) * scale / K
+
+    def vjp(P, T, w, stv):
+        state.load(P[0], T[1], T[2], T[3])
+        return state.energy_adjoint(basis, box, q_head, w=w, strain=stv)
+
+    lx = np.zeros((n, 3))
+    lv = np.zeros((n, 3))
+    lth = np.zeros(14)
+    J = 0.0
+    if objective == "energy":
+        r = vjp(records[-1], records[-1], None, None)
+        J = soup_potential(basis, records[-1][0], records[-1][1], records[-1][2], records[-1][3], box)[4]
+        lx += r["grad"]
+        lth += r["dU"]
+    for s in range(K - 1, -1, -1):
+        P = records[s]
+        T0 = start if s == 0 else records[s - 1]
+        last = s == K - 1
+        cw = a * dt / 2 + (0.0 if last else dt / 2)
+        done = obj is None
+        if not last and swept[s]:
+            r = vjp(P, P, (dt / 2) * lv, obj if obj is not None else st0)
+            lx -= r["hw"] + (obj if obj is not None else st0) * r["grad"]
+            lth += r["dFw"]
+            J += r["We"]
+            cw = a * dt / 2
+            done = True
+        if not done and (start_step + s) % bond_interval == 0:
+            r = vjp(P, P, None, obj)
+            lx -= r["hw"] + obj * r["grad"]
+            lth += r["dFw"]
+            J += r["We"]
+            done = True
+        stv = st0 if done else obj
+        r = vjp(P, T0, cw * lv, stv)
+        lx -= r["hw"] + stv * r["grad"]
+        lth += r["dFw"]
+        J += r["We"]
+        lv = a * lv + dt * lx
+    r = vjp(start, start, (dt / 2) * lv, st0)
+    lx -= r["hw"]
+    lth += r["dFw"]
+    return {"J": J, "dJ_dtheta": lth, "dJ_dx0": lx, "dJ_dv0": lv}
+
 
 def pcg_soup(v: int) -> int:
     return int(lib().or_pcg_soup(C.c_uint32(v)))
@@ -256,6 +344,22 @@ def soup_potential(basis, positions, bond_slots, center_link, charges, box):
     lib().or_soup_potential(C.byref(basis), C.c_uint32(n), pos.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p),
                             None if cl is None else cl.ctypes.data_as(C.c_void_p),
                             None if ch is None else ch.ctypes.data_as(C.c_void_p), bx.ctypes.data_as(C.c_void_p), out)
+    return tuple(out)
+
+
+def soup_potential_d(basis, positions, bond_slots, center_link, charges, box):
+    """soup_potential with FP64 positions (x, y, z, kind per particle): for strain derivatives, whose
+    scaled coordinates f32 would round."""
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1)
+    n = pos.size // 4
+    bs = np.ascontiguousarray(bond_slots, dtype=np.uint32).reshape(-1)
+    cl = None if center_link is None else np.ascontiguousarray(center_link, dtype=np.uint32)
+    ch = None if charges is None else np.ascontiguousarray(charges, dtype=np.float32)
+    bx = np.ascontiguousarray(box, dtype=np.float64)
+    out = (C.c_double * 5)()
+    lib().or_soup_potential_d(C.byref(basis), C.c_uint32(n), pos.ctypes.data_as(C.c_void_p), bs.ctypes.data_as(C.c_void_p),
+                              None if cl is None else cl.ctypes.data_as(C.c_void_p),
+                              None if ch is None else ch.ctypes.data_as(C.c_void_p), bx.ctypes.data_as(C.c_void_p), out)
     return tuple(out)
 
 

@@ -655,10 +655,10 @@ static inline double pot_round(double x) { /* JS Math.round: floor(x + 0.5) sema
   return r;
 }
 static inline double pot_mi(double d, double L) { return d - pot_round(d / L) * L; } /* electrostatics.ts:408-410 */
-static double pot_dist(const float* pos, uint32_t i, uint32_t j, const double* box) {
-  double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
-  double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
-  double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+static double pot_dist(const double* pos, uint32_t i, uint32_t j, const double* box) {
+  double dx = pot_mi(pos[4u * i] - pos[4u * j], box[0]);
+  double dy = pot_mi(pos[4u * i + 1] - pos[4u * j + 1], box[1]);
+  double dz = pot_mi(pos[4u * i + 2] - pos[4u * j + 2], box[2]);
   return sqrt(dx * dx + dy * dy + dz * dz);
 }
 static double pot_fene_v(const SoupPotentialBasis* b, double r) { /* soup-potential.ts:42-45 */
@@ -672,15 +672,16 @@ static double pot_attr_v(const SoupPotentialBasis* b, double r) { /* soup-potent
   return -b->epsilon * cc * cc;
 }
 
-void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos, const uint32_t* bond_slots,
-                       const uint32_t* center_link, const float* charges, const double* box, double out[5]) {
+/* positions in FP64 (x, y, z, kind per particle); or_soup_potential promotes f32 positions exactly */
+void or_soup_potential_d(const SoupPotentialBasis* b, uint32_t n, const double* pos, const uint32_t* bond_slots,
+                         const uint32_t* center_link, const float* charges, const double* box, double out[5]) {
   double cut2 = b->cutoff * b->cutoff, nonbonded = 0.0;
   for (uint32_t i = 0; i < n; ++i) { /* soup-potential.ts:199-240 */
     uint32_t ki = (uint32_t)pos[4u * i + 3];
     for (uint32_t j = i + 1; j < n; ++j) {
-      double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
-      double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
-      double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+      double dx = pot_mi(pos[4u * i] - pos[4u * j], box[0]);
+      double dy = pot_mi(pos[4u * i + 1] - pos[4u * j + 1], box[1]);
+      double dz = pot_mi(pos[4u * i + 2] - pos[4u * j + 2], box[2]);
       double r2 = dx * dx + dy * dy + dz * dz;
       if (r2 > cut2 || r2 <= 0.0) continue;
       double r = sqrt(r2);
@@ -729,9 +730,9 @@ void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos
       if ((uint32_t)pos[4u * i + 3] != b->charged_kind || charges[i] == 0.0f) continue;
       for (uint32_t j = i + 1; j < n; ++j) {
         if ((uint32_t)pos[4u * j + 3] != b->charged_kind || charges[j] == 0.0f) continue;
-        double dx = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
-        double dy = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
-        double dz = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+        double dx = pot_mi(pos[4u * i] - pos[4u * j], box[0]);
+        double dy = pot_mi(pos[4u * i + 1] - pos[4u * j + 1], box[1]);
+        double dz = pot_mi(pos[4u * i + 2] - pos[4u * j + 2], box[2]);
         double r2 = dx * dx + dy * dy + dz * dz;
         if (r2 >= ecut2) continue;
         double r = sqrt(r2);
@@ -746,6 +747,207 @@ void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos
   out[2] = bend;
   out[3] = tether;
   out[4] = nonbonded + fene + bend + tether;
+}
+
+void or_soup_potential(const SoupPotentialBasis* b, uint32_t n, const float* pos, const uint32_t* bond_slots,
+                       const uint32_t* center_link, const float* charges, const double* box, double out[5]) {
+  double* pd = (double*)malloc(4u * (size_t)n * sizeof(double));
+  for (size_t k = 0; k < 4u * (size_t)n; ++k) pd[k] = (double)pos[k];
+  or_soup_potential_d(b, n, pd, bond_slots, center_link, charges, box, out);
+  free(pd);
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* FP64 gradient, Hessian-vector product and parameter derivatives of soupPotential (the energy  */
+/* above, soup/src/soup-potential.ts:145-301), the CPU check of the product's adjoint. Each       */
+/* unordered pair term is visited once (list pairs and DH pairs with j > i, each bond from its    */
+/* lower end, each 1-3 bend from its middle bead, each mutual tether from its lower end) and     */
+/* applied to both of its particles. A pair term phi(r), r_vec = x_i - x_j under the minimum image, */
+/* e = r_vec / r, direction dw = w_i - w_j + diag(strain) r_vec, ew = e . dw:                     */
+/*   g_i += phi' e, g_j -= phi' e;                                                               */
+/*   hw_i += phi'' ew e + (phi' / r)(dw - ew e), hw_j -= the same;                               */
+/*   W_e -= phi' (e . diag(strain) r_vec);  d(w . F)/dtheta -= ew dphi'/dtheta.                   */
+/* theta: the 10 unordered class-pair depths, acid-soap, epsilon, head charge, kappa.            */
+/* Pairs come from the state's own Verlet and DH lists (or_rebuild first); the lists hold every  */
+/* pair within the basis cutoffs when those are at most listRange - skin.                        */
+/* ------------------------------------------------------------------------------------------- */
+typedef struct {
+  double* g;
+  double* hw;
+  double dU[14], dF[14], we;
+} AdjOut;
+static uint32_t adj_pidx(uint32_t a, uint32_t b) {
+  uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  static const uint32_t base[4] = {0u, 4u, 7u, 9u};
+  return base[lo] + (hi - lo);
+}
+/* one pair term (i, j) with derivatives d1, d2 at separation r_vec (length r); the caller adds the  */
+/* parameter parts through the returned ew                                                         */
+static double adj_pair(AdjOut* o, const double* w, const double* st, uint32_t i, uint32_t j, const double rv[3], double r,
+                       double d1, double d2) {
+  double e[3], dw[3], ew = 0.0, es = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    e[a] = rv[a] / r;
+    double sa = st[a] * rv[a];
+    dw[a] = (w ? w[3u * i + a] - w[3u * j + a] : 0.0) + sa;
+  }
+  for (int a = 0; a < 3; ++a) {
+    ew += e[a] * dw[a];
+    es += e[a] * st[a] * rv[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    double gi = d1 * e[a];
+    double hi = d2 * ew * e[a] + (d1 / r) * (dw[a] - ew * e[a]);
+    o->g[3u * i + a] += gi;
+    o->g[3u * j + a] -= gi;
+    o->hw[3u * i + a] += hi;
+    o->hw[3u * j + a] -= hi;
+  }
+  o->we -= d1 * es;
+  return ew;
+}
+static double adj_sep(const float* pos, uint32_t i, uint32_t j, const double* box, double rv[3]) {
+  rv[0] = pot_mi((double)pos[4u * i] - (double)pos[4u * j], box[0]);
+  rv[1] = pot_mi((double)pos[4u * i + 1] - (double)pos[4u * j + 1], box[1]);
+  rv[2] = pot_mi((double)pos[4u * i + 2] - (double)pos[4u * j + 2], box[2]);
+  return sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+}
+static void adj_fene_pair(AdjOut* o, const SoupPotentialBasis* b, const double* w, const double* st, const float* pos,
+                          const double* box, uint32_t i, uint32_t j) {
+  double rv[3], r = adj_sep(pos, i, j, box, rv);
+  if (!(r > 0.0)) return;
+  double x2 = (r / b->fene_r_inf) * (r / b->fene_r_inf);
+  adj_pair(o, w, st, i, j, rv, r, b->fene_k * r / (1.0 - x2), b->fene_k * (1.0 + x2) / ((1.0 - x2) * (1.0 - x2)));
+}
+
+void or_energy_adjoint(const OrState* s, uint32_t n, uint32_t list_cap, uint32_t es_cap, const SoupPotentialBasis* b,
+                       const double* box, double q_head, const double* w, const double* strain, double* g, double* hw,
+                       double* dU, double* dF, double* we) {
+  const double st0[3] = {0.0, 0.0, 0.0};
+  const double* st = strain ? strain : st0;
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  AdjOut* outs = (AdjOut*)calloc((size_t)nt, sizeof(AdjOut));
+  for (int t = 0; t < nt; ++t) {
+    outs[t].g = (double*)calloc(3u * (size_t)n, sizeof(double));
+    outs[t].hw = (double*)calloc(3u * (size_t)n, sizeof(double));
+  }
+  const double cut2 = b->cutoff * b->cutoff, eps = b->epsilon, z1 = M_PI / (2.0 * b->wc);
+  const int use_q = s->charge && b->use_charges;
+  const int es = use_q && b->es_A != 0.0;
+#pragma omp parallel
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    AdjOut* o = &outs[tid];
+#pragma omp for schedule(dynamic, 64)
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t ki = (uint32_t)s->pos[4u * i + 3];
+      const double qi = use_q ? (double)s->charge[i] : 0.0;
+      for (uint32_t k = 0; k < s->vcount[i]; ++k) { /* list pairs, soup-potential.ts:199-240 */
+        uint32_t j = s->vl[(size_t)i * list_cap + k];
+        if (j <= i) continue;
+        double rv[3], r = adj_sep(s->pos, i, j, box, rv);
+        if (r * r > cut2 || !(r > 0.0)) continue;
+        const uint32_t kj = (uint32_t)s->pos[4u * j + 3];
+        const double bb = (b->b_radius[ki] + b->b_radius[kj]) / 2.0;
+        if (r < b->wca_factor * bb) {
+          double u6 = pow(bb / r, 6.0), phi = eps * (2.0 * u6 - 1.0) * (2.0 * u6 - 1.0);
+          double d1 = -24.0 * eps * u6 * (2.0 * u6 - 1.0) / r, d2 = 24.0 * eps * u6 * (26.0 * u6 - 7.0) / (r * r);
+          double ew = adj_pair(o, w, st, i, j, rv, r, d1, d2);
+          o->dU[11] += phi / eps;
+          o->dF[11] -= ew * d1 / eps;
+        }
+        const uint32_t ci = b->cls[ki] & 3u, cj = b->cls[kj] & 3u;
+        const double qj = use_q ? (double)s->charge[j] : 0.0;
+        const int acid = b->acid_soap > 0.0 && use_q && ci == 1u && cj == 1u && ((qi == 0.0) != (qj == 0.0));
+        const double sc = b->attr[ci][cj] + (acid ? b->acid_soap : 0.0);
+        if (sc > 0.0 && r <= b->rc_attr + b->wc) {
+          double a0, a1, a2;
+          if (r < b->rc_attr) {
+            a0 = -eps; a1 = 0.0; a2 = 0.0;
+          } else {
+            double z = z1 * (r - b->rc_attr), cz = cos(z);
+            a0 = -eps * cz * cz;
+            a1 = eps * z1 * sin(2.0 * z);
+            a2 = 2.0 * eps * z1 * z1 * cos(2.0 * z);
+          }
+          double ew = adj_pair(o, w, st, i, j, rv, r, sc * a1, sc * a2);
+          uint32_t pi = adj_pidx(ci, cj);
+          o->dU[pi] += a0;
+          o->dF[pi] -= ew * a1;
+          if (acid) {
+            o->dU[10] += a0;
+            o->dF[10] -= ew * a1;
+          }
+          o->dU[11] += sc * a0 / eps;
+          o->dF[11] -= ew * sc * a1 / eps;
+        }
+      }
+      if (es && qi != 0.0 && ki == b->charged_kind) { /* esTotalEnergy pairs, electrostatics.ts:575-600 */
+        uint32_t h = 0, m = s->nh;
+        while (h < m && s->head_idx[h] != i) ++h;
+        if (h < m) {
+          const double rc = b->es_cutoff, kap = b->es_kappa, erc = b->es_A * exp(-kap * rc);
+          for (uint32_t k = 0; k < s->es_count[h]; ++k) {
+            uint32_t j = s->es_list[(size_t)h * es_cap + k];
+            if (j <= i || s->charge[j] == 0.0f) continue;
+            double rv[3], r = adj_sep(s->pos, i, j, box, rv);
+            if (r >= rc || !(r > 0.0)) continue;
+            double qq = qi * (double)s->charge[j];
+            double u1 = b->es_A * exp(-kap * r) / r, k1 = kap + 1.0 / r;
+            double e0 = u1 - b->es_shiftU + b->es_shiftF * (r - rc);
+            double e1 = -u1 * k1 + b->es_shiftF, e2 = u1 * k1 * k1 + u1 / (r * r);
+            double ew = adj_pair(o, w, st, i, j, rv, r, qq * e1, qq * e2);
+            o->dU[12] += 2.0 * qq * e0 / q_head;
+            o->dF[12] -= ew * 2.0 * qq * e1 / q_head;
+            o->dU[13] += qq * (-r * u1 + erc - kap * erc * (r - rc));
+            o->dF[13] -= ew * qq * (u1 * r * kap - kap * erc);
+          }
+        }
+      }
+      for (int k = 0; k < 3; ++k) { /* FENE once per bond, soup-potential.ts:254-256 */
+        uint32_t j = s->bond_slots[3u * i + k];
+        if (j != NONE && j > i) adj_fene_pair(o, b, w, st, s->pos, box, i, j);
+      }
+      for (int a = 0; a < 3; ++a) { /* 1-3 bends with i as the middle bead, :258-267 */
+        uint32_t pa = s->bond_slots[3u * i + a];
+        if (pa == NONE) continue;
+        for (int c2 = a + 1; c2 < 3; ++c2) {
+          uint32_t pc = s->bond_slots[3u * i + c2];
+          if (pc == NONE) continue;
+          double rv[3], r = adj_sep(s->pos, pa, pc, box, rv);
+          if (r > 0.0) adj_pair(o, w, st, pa, pc, rv, r, b->bend_k * (r - b->bend_r0), b->bend_k);
+        }
+      }
+      if (s->center_link && b->use_center_link) { /* tether once per mutual link, :269-276 */
+        uint32_t owner = s->center_link[i];
+        if (owner != NONE && owner > i) adj_fene_pair(o, b, w, st, s->pos, box, i, owner);
+      }
+    }
+  }
+  memset(g, 0, 3u * (size_t)n * sizeof(double));
+  if (hw) memset(hw, 0, 3u * (size_t)n * sizeof(double));
+  for (int p = 0; p < 14; ++p) dU[p] = dF[p] = 0.0;
+  *we = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    for (size_t k = 0; k < 3u * (size_t)n; ++k) {
+      g[k] += outs[t].g[k];
+      if (hw) hw[k] += outs[t].hw[k];
+    }
+    for (int p = 0; p < 14; ++p) {
+      dU[p] += outs[t].dU[p];
+      dF[p] += outs[t].dF[p];
+    }
+    *we += outs[t].we;
+    free(outs[t].g);
+    free(outs[t].hw);
+  }
+  free(outs);
 }
 
 /* ------------------------------------------------------------------------------------------- */

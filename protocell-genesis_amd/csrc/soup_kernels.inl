@@ -2126,10 +2126,20 @@ __global__ __launch_bounds__(1024) void k_potential_sum(const double4* part, uin
 // dw = w_i - w_j; parameters take each unordered pair once (the lower particle index), as the
 // potential does. The pairs are those of soupPotential: list pairs within basis->cutoff, bonds,
 // 1-3 bends through each centre, centre-link tethers, and DH pairs of charged heads.
+//
+// A diagonal strain e = (ex, ey, ez) adds diag(e) r_ij to every pair's dw, r_ij = x_i - x_j under
+// the minimum image: the direction of a homogeneous deformation. Every term is a pair term, so the
+// strain virial W_e = sum over pairs of (diag(e) r_ij) . f_ij (f_ij = -phi' e_ij, the force on i)
+// then has dW_e/dx_i = -diag(e) g_i - (H_e)_i, where H_e is the Hessian product the strain adds to
+// hw, and dW_e/dtheta is what it adds to d(w . F)/dtheta; W_e itself is reduced as one more value.
 // ================================================================================================
 #define ADJ_P 14u  // 10 class pairs, acid-soap, epsilon, head charge, kappa
+#define ADJ_NPART (2u * ADJ_P + 1u)  // per-particle partials: dU, d(w . F), W_e
 struct AdjAcc {
-  double g[3], hw[3], dU[ADJ_P], dF[ADJ_P];
+  double g[3], hw[3], dU[ADJ_P], dF[ADJ_P], we;
+};
+struct AdjStrain {
+  double e[3];
 };
 __device__ __forceinline__ uint32_t adj_attr_index(uint32_t a, uint32_t b) {
   const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
@@ -2160,7 +2170,10 @@ __device__ __forceinline__ double adj_geom(const SoupConfig& c, float4 xi, float
   e[2] = pot_mi((double)xi.z - (double)xj.z, c.box[2]);
   return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
 }
-__device__ __forceinline__ void adj_dw(const double* w, uint32_t idi, uint32_t idj, double dw[3]) {
+// dw = w_i - w_j + diag(strain) r_ij (e: the unit vector from j to i, r its length); es = the strain
+// part's projection on e
+__device__ __forceinline__ void adj_dw(const double* w, const AdjStrain& st, uint32_t idi, uint32_t idj, const double e[3],
+                                       double r, double dw[3], double& es) {
   if (w) {
     dw[0] = w[3u * idi] - w[3u * idj];
     dw[1] = w[3u * idi + 1u] - w[3u * idj + 1u];
@@ -2168,23 +2181,33 @@ __device__ __forceinline__ void adj_dw(const double* w, uint32_t idi, uint32_t i
   } else {
     dw[0] = dw[1] = dw[2] = 0.0;
   }
+  es = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double d = st.e[a] * e[a] * r;
+    dw[a] += d;
+    es += e[a] * d;
+  }
 }
 // a bonded pair term (FENE: fene = true; 1-3 bend otherwise) on particle i
+// (W_e takes each bonded pair term once, from its lower particle index)
 __device__ __forceinline__ void adj_bonded(const SoupConfig& c, const SoupPotentialBasis& pb, float4 xi, float4 xj,
-                                           uint32_t idi, uint32_t idj, const double* w, bool fene, AdjAcc& A) {
+                                           uint32_t idi, uint32_t idj, const double* w, const AdjStrain& st, bool fene,
+                                           AdjAcc& A) {
   double e[3];
   const double r = sqrt(adj_geom(c, xi, xj, e));
   if (!(r > 0.0)) return;
   e[0] /= r; e[1] /= r; e[2] /= r;
-  double dw[3];
-  adj_dw(w, idi, idj, dw);
+  double dw[3], es;
+  adj_dw(w, st, idi, idj, e, r, dw, es);
   const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
-  if (fene) adj_pos(A, adj_fene_d1(pb, r), adj_fene_d2(pb, r), e, r, dw, ew);
-  else adj_pos(A, pb.bend_k * (r - pb.bend_r0), pb.bend_k, e, r, dw, ew);
+  const double d1 = fene ? adj_fene_d1(pb, r) : pb.bend_k * (r - pb.bend_r0);
+  adj_pos(A, d1, fene ? adj_fene_d2(pb, r) : pb.bend_k, e, r, dw, ew);
+  if (idj > idi) A.we -= es * d1;
 }
 
 __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, SoupPotentialBasis pb, int es, const double* w,
-                                                        double* g_out, double* hw_out, double* part) {
+                                                        AdjStrain st, double* g_out, double* hw_out, double* part) {
   const uint32_t lane = threadIdx.x & (WAVE - 1), wv = uni(threadIdx.x / WAVE);
   const uint32_t t = uni(blockIdx.x * (blockDim.x / WAVE) + wv);
   if (t >= c.n) return;
@@ -2200,6 +2223,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
   for (int a = 0; a < 3; ++a) A.g[a] = A.hw[a] = 0.0;
 #pragma unroll
   for (uint32_t p = 0; p < ADJ_P; ++p) A.dU[p] = A.dF[p] = 0.0;
+  A.we = 0.0;
   const uint32_t cnt = uni(b.vcount[t]), cn = uni(b.vnear[t]);
   const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   for (uint32_t s = lane; s < cnt; s += WAVE) {
@@ -2211,10 +2235,10 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
     const double r = sqrt(r2);
     e[0] /= r; e[1] /= r; e[2] /= r;
     const uint32_t idj = b.id[u];
-    double dw[3];
-    adj_dw(w, idi, idj, dw);
+    double dw[3], ews;
+    adj_dw(w, st, idi, idj, e, r, dw, ews);
     const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
-    const bool once = idj > idi;  // parameters: each unordered pair once
+    const bool once = idj > idi;  // parameters and W_e: each unordered pair once
     const uint32_t kj = (uint32_t)xj.w & (SOUP_MAX_KINDS - 1u);
     const double bb = (pb.b_radius[ki] + pb.b_radius[kj]) / 2.0;
     if (r < pb.wca_factor * bb) {
@@ -2226,6 +2250,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
       if (once) {
         A.dU[11] += phi / eps;
         A.dF[11] -= ew * d1 / eps;
+        A.we -= ews * d1;
       }
     }
     const uint32_t ci = pb.cls[ki] & 3u, cj = pb.cls[kj] & 3u;
@@ -2244,6 +2269,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
       }
       adj_pos(A, sc * a1, sc * a2, e, r, dw, ew);
       if (once) {
+        A.we -= ews * sc * a1;
         const uint32_t pi = adj_attr_index(ci, cj);
         A.dU[pi] += a0;
         A.dF[pi] -= ew * a1;
@@ -2263,18 +2289,18 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
       const uint32_t p = sl[k];
       if (p == SOUP_NONE) continue;
       const uint32_t up = b.inv[p];
-      adj_bonded(c, pb, xi, b.pos[up], idi, p, w, true, A);
+      adj_bonded(c, pb, xi, b.pos[up], idi, p, w, st, true, A);
       // 1-3 bends through partner p: this particle and p's other partners
       const uint4 ms = b.bs[up];
       const uint32_t ml[3] = {ms.x, ms.y, ms.z};
       for (int q = 0; q < 3; ++q) {
         const uint32_t o = ml[q];
         if (o == SOUP_NONE || o == idi) continue;
-        adj_bonded(c, pb, xi, b.pos[b.inv[o]], idi, o, w, false, A);
+        adj_bonded(c, pb, xi, b.pos[b.inv[o]], idi, o, w, st, false, A);
       }
     }
     const uint32_t owner = b.clink[t];
-    if (pb.use_center_link && owner != SOUP_NONE) adj_bonded(c, pb, xi, b.pos[b.inv[owner]], idi, owner, w, true, A);
+    if (pb.use_center_link && owner != SOUP_NONE) adj_bonded(c, pb, xi, b.pos[b.inv[owner]], idi, owner, w, st, true, A);
   }
   const uint32_t hh = es ? uni(b.head_of_slot[t]) : SOUP_NONE;
   if (hh != SOUP_NONE && qi != 0.0f) {
@@ -2293,8 +2319,8 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
       if (!(r > 0.0)) continue;
       e[0] /= r; e[1] /= r; e[2] /= r;
       const uint32_t idj = b.head_id[gh];
-      double dw[3];
-      adj_dw(w, idi, idj, dw);
+      double dw[3], ews;
+      adj_dw(w, st, idi, idj, e, r, dw, ews);
       const double ew = e[0] * dw[0] + e[1] * dw[1] + e[2] * dw[2];
       const double qq = (double)qi * (double)xq.w;
       const double u1 = (pb.es_A * exp(-kap * r)) / r, k1 = kap + 1.0 / r;
@@ -2303,6 +2329,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
       const double e2 = u1 * k1 * k1 + u1 / r2;
       adj_pos(A, qq * e1, qq * e2, e, r, dw, ew);
       if (idj > idi) {
+        A.we -= ews * qq * e1;
         A.dU[12] += 2.0 * qq * e0 / c.ph_q;
         A.dF[12] -= ew * 2.0 * qq * e1 / c.ph_q;
         A.dU[13] += qq * (-r * u1 + erc - kap * erc * (r - rc));
@@ -2320,6 +2347,7 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
     A.dU[p] = wave_sum_d_dpp(A.dU[p]);
     A.dF[p] = wave_sum_d_dpp(A.dF[p]);
   }
+  A.we = wave_sum_d_dpp(A.we);
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -2328,18 +2356,19 @@ __global__ __launch_bounds__(256) void k_energy_adjoint(SoupConfig c, DevBuf b, 
     }
 #pragma unroll
     for (uint32_t p = 0; p < ADJ_P; ++p) {
-      part[(size_t)t * (2u * ADJ_P) + p] = A.dU[p];
-      part[(size_t)t * (2u * ADJ_P) + ADJ_P + p] = A.dF[p];
+      part[(size_t)t * ADJ_NPART + p] = A.dU[p];
+      part[(size_t)t * ADJ_NPART + ADJ_P + p] = A.dF[p];
     }
+    part[(size_t)t * ADJ_NPART + 2u * ADJ_P] = A.we;
   }
 }
 
-// fixed-order sum of the per-particle parameter partials (k_potential_sum's scheme, 2 * ADJ_P values)
+// fixed-order sum of the per-particle partials (k_potential_sum's scheme, ADJ_NPART values)
 __global__ __launch_bounds__(1024) void k_adjoint_sum(const double* part, uint32_t n, double* out) {
   __shared__ double s[1024];
-  for (uint32_t p = 0; p < 2u * ADJ_P; ++p) {
+  for (uint32_t p = 0; p < ADJ_NPART; ++p) {
     double a = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a += part[(size_t)i * (2u * ADJ_P) + p];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) a += part[(size_t)i * ADJ_NPART + p];
     s[threadIdx.x] = a;
     __syncthreads();
     for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
@@ -2349,6 +2378,74 @@ __global__ __launch_bounds__(1024) void k_adjoint_sum(const double* part, uint32
     if (threadIdx.x == 0) out[p] = s[0];
     __syncthreads();
   }
+}
+
+// ================================================================================================
+// Trajectory-segment adjoint helpers: states move between a system's slot order and particle order
+// (checkpoints and per-step records), and the reverse sweep's adjoint vectors (FP64, particle
+// order, 3 values per particle) are updated on the device.
+// ================================================================================================
+struct StateRefs {  // particle-order arrays of one state; a null member is skipped
+  float4 *pos, *vel, *force;
+  float* charge;
+  uint4* bs;
+  uint32_t *clink, *cheld, *frozen, *brng, *trng;
+};
+// slot order -> particle order
+__global__ __launch_bounds__(256) void k_state_gather(uint32_t n, DevBuf b, StateRefs s) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t t = b.inv[p];
+  if (s.pos) s.pos[p] = b.pos[t];
+  if (s.vel) s.vel[p] = b.vel[t];
+  if (s.force) s.force[p] = b.force[t];
+  if (s.charge) s.charge[p] = b.charge[t];
+  if (s.bs) s.bs[p] = b.bs[t];
+  if (s.clink) s.clink[p] = b.clink[t];
+  if (s.cheld) s.cheld[p] = b.cheld[t];
+  if (s.frozen) s.frozen[p] = b.frozen[t];
+  if (s.brng) s.brng[p] = b.brng[t];
+  if (s.trng) s.trng[p] = b.trng[t];
+}
+// particle order -> the system's current slot order
+__global__ __launch_bounds__(256) void k_state_scatter(uint32_t n, DevBuf b, StateRefs s) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t p = b.id[t];
+  if (s.pos) b.pos[t] = s.pos[p];
+  if (s.vel) b.vel[t] = s.vel[p];
+  if (s.force) b.force[t] = s.force[p];
+  if (s.charge) b.charge[t] = s.charge[p];
+  if (s.bs) b.bs[t] = s.bs[p];
+  if (s.clink) b.clink[t] = s.clink[p];
+  if (s.cheld) b.cheld[t] = s.cheld[p];
+  if (s.frozen) b.frozen[t] = s.frozen[p];
+  if (s.brng) b.brng[t] = s.brng[p];
+  if (s.trng) b.trng[t] = s.trng[p];
+}
+// w = cw * lv
+__global__ __launch_bounds__(256) void k_seg_w(uint32_t m, const double* lv, double cw, double* w) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) w[i] = cw * lv[i];
+}
+// lx += -hw - diag(e) g: the vector-Jacobian product of the force (-H w) and, with the strain e of
+// the objective (zero without), the strain virial's position gradient -diag(e) g - H_e
+__global__ __launch_bounds__(256) void k_seg_lx(uint32_t n, double* lx, const double* hw, const double* g, AdjStrain st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) lx[3u * i + a] -= hw[3u * i + a] + st.e[a] * g[3u * i + a];
+}
+// the velocity adjoint of one step backwards: lv = a lv + dt lx
+__global__ __launch_bounds__(256) void k_seg_lv(uint32_t m, double* lv, const double* lx, double a, double dt) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) lv[i] = a * lv[i] + dt * lx[i];
+}
+// acc[0..ADJ_P) += the d(w . F)/dtheta partials (or the dU/dtheta ones: du = 1), acc[ADJ_P] += W_e
+__global__ void k_seg_acc(const double* out, double* acc, int du) {
+  const uint32_t p = threadIdx.x;
+  if (p < ADJ_P) acc[p] += out[(du ? 0u : ADJ_P) + p];
+  if (p == ADJ_P) acc[ADJ_P] += out[2u * ADJ_P];
 }
 
 // ================================================================================================

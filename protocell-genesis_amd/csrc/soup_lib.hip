@@ -1671,38 +1671,55 @@ int soup_potential(SoupHandle* h, const SoupPotentialBasis* pb, double out[5]) {
   return sync(h);
 }
 
-int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const double* w, double* grad, double* hw,
-                        double* dU_dtheta, double* dFw_dtheta) {
-  if (!h) return SOUP_E_INVALID;
-  if (!pb || !grad) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: null basis or gradient output");
-  if ((hw || dFw_dtheta) && !w) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: the w-products need a direction w");
+// the argument and list checks of an adjoint launch on h's current lists; `who` names the entry point
+static int adjoint_ready(SoupHandle* h, const char* who, const SoupPotentialBasis* pb, bool& es) {
   const double near_ok = (double)h->c.list_range - (double)h->c.skin;
   if (pb->cutoff > near_ok + 1e-9)
-    return fail(h, SOUP_E_INVALID, fmt("soup_energy_adjoint: pair cutoff %.6f exceeds the Verlet list's guaranteed range %.6f",
-                                       pb->cutoff, near_ok));
-  const bool es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
+    return fail(h, SOUP_E_INVALID, fmt("%s: pair cutoff %.6f exceeds the Verlet list's guaranteed range %.6f", who, pb->cutoff,
+                                       near_ok));
+  es = es_on(h) && pb->use_charges && pb->es_A != 0.0;
   if (es && pb->es_cutoff > (double)h->c.es_list_range - (double)h->c.skin + 1e-6)
-    return fail(h, SOUP_E_INVALID, fmt("soup_energy_adjoint: DH cutoff %.6f exceeds the DH list's guaranteed range %.6f",
-                                       pb->es_cutoff, (double)h->c.es_list_range - (double)h->c.skin));
-  if (es && h->c.ph_q == 0.0) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint: chargeDeprotonated is 0");
-  int r = sync(h);
-  if (r) return r;
-  r = assert_lists(h);
-  if (r) return r;
-  const uint32_t n = h->n;
+    return fail(h, SOUP_E_INVALID, fmt("%s: DH cutoff %.6f exceeds the DH list's guaranteed range %.6f", who, pb->es_cutoff,
+                                       (double)h->c.es_list_range - (double)h->c.skin));
+  if (es && h->c.ph_q == 0.0) return fail(h, SOUP_E_INVALID, fmt("%s: chargeDeprotonated is 0", who));
   if (!h->adj_g) {  // sized for the capacity: the live count may grow back after an evaporation
     DA(h, h->adj_w, (size_t)h->cap * 3);
     DA(h, h->adj_g, (size_t)h->cap * 3);
     DA(h, h->adj_hw, (size_t)h->cap * 3);
-    DA(h, h->adj_part, (size_t)h->cap * 2 * ADJ_P);
-    DA(h, h->adj_out, 2 * ADJ_P);
+    DA(h, h->adj_part, (size_t)h->cap * ADJ_NPART);
+    DA(h, h->adj_out, ADJ_NPART);
   }
-  if (w) HC(h, hipMemcpyAsync(h->adj_w, w, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+// one adjoint launch on h's current state and lists (w: device, particle order, or null): gradient
+// into adj_g, the (w + strain) Hessian product into adj_hw, the reduced partials into adj_out
+static void adjoint_launch(SoupHandle* h, const SoupPotentialBasis* pb, bool es, const double* w_dev, const AdjStrain& st) {
   if (es) hipLaunchKernelGGL(k_head_refresh, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
-  hipLaunchKernelGGL(k_energy_adjoint, dim3(blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, es ? 1 : 0,
-                     w ? (const double*)h->adj_w : nullptr, h->adj_g, w ? h->adj_hw : nullptr, h->adj_part);
-  hipLaunchKernelGGL(k_adjoint_sum, dim3(1), dim3(1024), 0, h->stream, h->adj_part, n, h->adj_out);
-  double sums[2 * ADJ_P];
+  hipLaunchKernelGGL(k_energy_adjoint, dim3(blocks((uint64_t)h->n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, *pb, es ? 1 : 0,
+                     w_dev, st, h->adj_g, h->adj_hw, h->adj_part);
+  hipLaunchKernelGGL(k_adjoint_sum, dim3(1), dim3(1024), 0, h->stream, h->adj_part, h->n, h->adj_out);
+}
+
+static int energy_adjoint(SoupHandle* h, const char* who, const SoupPotentialBasis* pb, const double* strain, const double* w,
+                          double* grad, double* hw, double* dU_dtheta, double* dFw_dtheta, double* W_e) {
+  if (!h) return SOUP_E_INVALID;
+  if (!pb || !grad) return fail(h, SOUP_E_INVALID, fmt("%s: null basis or gradient output", who));
+  if ((hw || dFw_dtheta) && !w && !strain) return fail(h, SOUP_E_INVALID, fmt("%s: the w-products need a direction w", who));
+  bool es = false;
+  int r = adjoint_ready(h, who, pb, es);
+  if (r) return r;
+  r = sync(h);
+  if (r) return r;
+  r = assert_lists(h);
+  if (r) return r;
+  const uint32_t n = h->n;
+  AdjStrain st{{0.0, 0.0, 0.0}};
+  if (strain)
+    for (int a = 0; a < 3; ++a) st.e[a] = strain[a];
+  if (w) HC(h, hipMemcpyAsync(h->adj_w, w, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  adjoint_launch(h, pb, es, w ? (const double*)h->adj_w : nullptr, st);
+  double sums[ADJ_NPART];
   HC(h, hipMemcpyAsync(grad, h->adj_g, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   if (hw) HC(h, hipMemcpyAsync(hw, h->adj_hw, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HC(h, hipMemcpyAsync(sums, h->adj_out, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
@@ -1712,6 +1729,330 @@ int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const doubl
     if (dU_dtheta) dU_dtheta[p] = sums[p];
     if (dFw_dtheta) dFw_dtheta[p] = sums[ADJ_P + p];
   }
+  if (W_e) *W_e = sums[2 * ADJ_P];
+  return 0;
+}
+
+int soup_energy_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const double* w, double* grad, double* hw,
+                        double* dU_dtheta, double* dFw_dtheta) {
+  return energy_adjoint(h, "soup_energy_adjoint", pb, nullptr, w, grad, hw, dU_dtheta, dFw_dtheta, nullptr);
+}
+
+int soup_energy_adjoint_strain(SoupHandle* h, const SoupPotentialBasis* pb, const double strain[3], const double* w,
+                               double* grad, double* hw, double* dU_dtheta, double* dFw_dtheta, double* W_e) {
+  if (h && !strain) return fail(h, SOUP_E_INVALID, "soup_energy_adjoint_strain: null strain");
+  return energy_adjoint(h, "soup_energy_adjoint_strain", pb, strain, w, grad, hw, dU_dtheta, dFw_dtheta, W_e);
+}
+
+}  // extern "C": the segment adjoint's C++ helpers follow
+
+// ---- trajectory-segment adjoint with device checkpoints ---------------------------------------------
+// Device buffers of one call, freed when it returns.
+struct DevScratch {
+  std::vector<void*> ptrs;
+  template <typename T>
+  hipError_t get(T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+  }
+  ~DevScratch() {
+    for (void* q : ptrs) (void)hipFree(q);
+  }
+};
+// a full checkpoint (particle order) and the host counters the steps after it depend on
+struct SegCheckpoint {
+  StateRefs st{};
+  uint32_t* events = nullptr;
+  uint32_t* desorb = nullptr;
+  uint64_t global_step = 0, next_sweep_at = 0;
+  uint32_t prng = 0, sweeps = 0;
+  uint64_t rebuilds = 0;
+};
+static StateRefs state_part(const StateRefs& s, bool topo_only) {
+  StateRefs r{};
+  r.pos = s.pos;
+  r.charge = s.charge;
+  r.bs = s.bs;
+  r.clink = s.clink;
+  if (!topo_only) {
+    r.vel = s.vel; r.force = s.force; r.cheld = s.cheld; r.frozen = s.frozen; r.brng = s.brng; r.trng = s.trng;
+  }
+  return r;
+}
+static void seg_save(SoupHandle* h, SegCheckpoint& ck) {
+  hipLaunchKernelGGL(k_state_gather, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->n, h->b, ck.st);
+  const size_t ev = 2u * (size_t)h->c.n_rules;
+  if (ev) (void)hipMemcpyAsync(ck.events, h->b.events, ev * 4, hipMemcpyDeviceToDevice, h->stream);
+  (void)hipMemcpyAsync(ck.desorb, h->b.desorb, 8, hipMemcpyDeviceToDevice, h->stream);
+  ck.global_step = h->global_step;
+  ck.next_sweep_at = h->next_sweep_at;
+  ck.prng = h->prng;
+  ck.sweeps = h->sweeps;
+  ck.rebuilds = h->rebuilds;
+}
+static void seg_restore(SoupHandle* r, const SegCheckpoint& ck) {
+  hipLaunchKernelGGL(k_state_scatter, dim3(blocks(r->n, 256)), dim3(256), 0, r->stream, r->n, r->b, ck.st);
+  const size_t ev = 2u * (size_t)r->c.n_rules;
+  if (ev) (void)hipMemcpyAsync(r->b.events, ck.events, ev * 4, hipMemcpyDeviceToDevice, r->stream);
+  (void)hipMemcpyAsync(r->b.desorb, ck.desorb, 8, hipMemcpyDeviceToDevice, r->stream);
+  r->global_step = ck.global_step;
+  r->next_sweep_at = ck.next_sweep_at;
+  r->prng = ck.prng;
+  r->sweeps = ck.sweeps;
+  r->rebuilds = ck.rebuilds;
+}
+// the steps [s0, s1) of a K-step call that began at call_start, with soup_step's chunk ends (after
+// every 1000th step of the call and after its last): per step (record) or in runs (forward).
+// swept[s] = 1 where the chunk end after step s ran a constant-pH sweep; rec (record mode) receives
+// the state after each step, rec[i] for step s0 + i - 1.
+static int seg_run(SoupHandle* h, uint32_t s0, uint32_t s1, uint32_t K, uint64_t call_start, std::vector<uint8_t>& swept,
+                   const std::vector<StateRefs>* rec) {
+  uint32_t s = s0;
+  while (s < s1) {
+    const uint32_t chunk_last = std::min(K, (s / 1000u + 1u) * 1000u) - 1u;  // the chunk's last step
+    const uint32_t e = std::min(s1 - 1u, chunk_last);
+    if (rec) {
+      for (uint32_t q = s; q <= e; ++q) {
+        one_step(h);
+        if (q != chunk_last)
+          hipLaunchKernelGGL(k_state_gather, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->n, h->b, (*rec)[q - s0 + 1]);
+      }
+    } else {
+      const int r = chunk_steps(h, e - s + 1u);
+      if (r) return r;
+    }
+    if (e == chunk_last) {
+      const uint32_t sw0 = h->sweeps;
+      const int r = chunk_end(h, call_start + (uint64_t)(chunk_last / 1000u) * 1000u);
+      if (r) return r;
+      const uint8_t sw = h->sweeps != sw0 ? 1 : 0;
+      if (rec) {
+        if (sw != swept[e])
+          return fail(h, SOUP_E_DEVICE, fmt("soup_segment_adjoint: the recomputed step %u disagrees with the forward pass "
+                                            "on its constant-pH sweep", e));
+        hipLaunchKernelGGL(k_state_gather, dim3(blocks(h->n, 256)), dim3(256), 0, h->stream, h->n, h->b, (*rec)[e - s0 + 1]);
+      } else {
+        swept[e] = sw;
+      }
+    }
+    s = e + 1u;
+  }
+  return 0;
+}
+// a system holding h's current state and configuration (the replay system of the reverse sweep)
+static int seg_clone(SoupHandle* h, SoupHandle** out) {
+  const size_t n = h->n;
+  std::vector<float> pos(4 * n), vel(4 * n), ch(n);
+  std::vector<uint32_t> bs(3 * n), cl(n), held(n), fr(n), brng(n), trng(n), ev(std::max<size_t>(2u * h->c.n_rules, 1)), de(2);
+  int r = soup_read(h, SOUP_FIELD_POS, pos.data(), 16 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_VEL, vel.data(), 16 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_CHARGE, ch.data(), 4 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_BOND_SLOTS, bs.data(), 12 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_CENTER_LINK, cl.data(), 4 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_CENTER_HELD, held.data(), 4 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_FROZEN, fr.data(), 4 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_BOND_RNG, brng.data(), 4 * n);
+  if (!r) r = soup_read(h, SOUP_FIELD_THERMO_RNG, trng.data(), 4 * n);
+  if (!r && h->c.n_rules) r = soup_read(h, SOUP_FIELD_EVENTS, ev.data(), 8 * (size_t)h->c.n_rules);
+  if (!r) r = soup_read(h, SOUP_FIELD_DESORB, de.data(), 8);
+  if (r) return r;
+  SoupInitState st{};
+  st.positions = pos.data();
+  st.velocities = vel.data();
+  st.bond_slots = bs.data();
+  st.center_link = cl.data();
+  st.center_held = held.data();
+  st.frozen = fr.data();
+  st.bond_rng = brng.data();
+  st.thermo_rng = trng.data();
+  st.charges = ch.data();
+  st.events = ev.data();
+  st.desorb_events = de.data();
+  st.global_step = h->global_step;
+  st.next_sweep_at = h->next_sweep_at;
+  st.protonation_rng = h->prng;
+  SoupConfig c = h->c;
+  c.n = h->n;
+  r = soup_create(&c, &st, out);
+  if (r) return fail(h, r, "soup_segment_adjoint: the replay system: " + std::string(soup_last_error(nullptr)));
+  (*out)->use_graphs = false;
+  return 0;
+}
+
+extern "C" {
+
+int soup_segment_adjoint(SoupHandle* h, const SoupPotentialBasis* pb, const SoupSegmentSpec* sp, double* J, double* dJ_dtheta,
+                         double* dJ_dx0, double* dJ_dv0) {
+  static const char* who = "soup_segment_adjoint";
+  if (!h) return SOUP_E_INVALID;
+  if (!pb || !sp || !J || !dJ_dtheta) return fail(h, SOUP_E_INVALID, fmt("%s: null argument", who));
+  const SoupConfig& c = h->c;
+  const uint32_t K = sp->steps, C = sp->checkpoint_every, E = c.rebuild_every;
+  if (K == 0 || C == 0) return fail(h, SOUP_E_INVALID, fmt("%s: steps and checkpoint_every must be >= 1", who));
+  if (sp->objective > SOUP_OBJ_MEAN_STRAIN_VIRIAL) return fail(h, SOUP_E_INVALID, fmt("%s: unknown objective %u", who, sp->objective));
+  if (drift_mode(h)) return fail(h, SOUP_E_INVALID, fmt("%s: needs the reference's rebuild cadence (rebuild = cadence)", who));
+  // a recomputed stretch starts where a cadence step rebuilds anyway, so it rebuilds the lists the
+  // forward pass built and reproduces it bit for bit from the checkpointed state
+  if (h->global_step % E != 0 || C % E != 0)
+    return fail(h, SOUP_E_INVALID, fmt("%s: the start step (%llu) and checkpoint_every (%u) must be multiples of rebuildEvery (%u)", who,
+                                       (unsigned long long)h->global_step, C, E));
+  if (h->global_step > 0 && (h->global_step - 1) % c.bond_interval == 0)
+    return fail(h, SOUP_E_INVALID, fmt("%s: the segment starts right after a bond Monte Carlo step (its force buffer holds the "
+                                       "previous topology's force); start one step later", who));
+  {
+    std::vector<uint32_t> fr(h->n);
+    int r = soup_read(h, SOUP_FIELD_FROZEN, fr.data(), 4 * (size_t)h->n);
+    if (r) return r;
+    for (uint32_t v : fr)
+      if (v) return fail(h, SOUP_E_INVALID, fmt("%s: frozen particles are not differentiated", who));
+  }
+  bool es = false;
+  int r = adjoint_ready(h, who, pb, es);
+  if (r) return r;
+  r = sync(h);
+  if (r) return r;
+  const uint32_t n = h->n, nseg = (K + C - 1) / C;
+  const size_t ev = 2u * (size_t)c.n_rules;
+  const uint64_t call_start = h->global_step;
+  DevScratch ds;
+  auto alloc_state = [&](StateRefs& s, bool topo_only) -> hipError_t {
+    hipError_t e = ds.get(&s.pos, n);
+    if (!e) e = ds.get(&s.charge, n);
+    if (!e) e = ds.get(&s.bs, n);
+    if (!e) e = ds.get(&s.clink, n);
+    if (!e && !topo_only) {
+      e = ds.get(&s.vel, n);
+      if (!e) e = ds.get(&s.force, n);
+      if (!e) e = ds.get(&s.cheld, n);
+      if (!e) e = ds.get(&s.frozen, n);
+      if (!e) e = ds.get(&s.brng, n);
+      if (!e) e = ds.get(&s.trng, n);
+    }
+    return e;
+  };
+  std::vector<SegCheckpoint> ck(nseg);
+  std::vector<StateRefs> rec(C + 1u);
+  double *lx = nullptr, *lv = nullptr, *acc = nullptr;
+  hipError_t he = hipSuccess;
+  for (auto& k : ck) {
+    if (!he) he = alloc_state(k.st, false);
+    if (!he) he = ds.get(&k.events, std::max<size_t>(ev, 1));
+    if (!he) he = ds.get(&k.desorb, 2);
+  }
+  for (auto& q : rec)
+    if (!he) he = alloc_state(q, true);
+  if (!he) he = ds.get(&lx, 3 * (size_t)n);
+  if (!he) he = ds.get(&lv, 3 * (size_t)n);
+  if (!he) he = ds.get(&acc, ADJ_P + 1);
+  if (he)
+    return fail(h, SOUP_E_NOMEM, fmt("%s: device buffers for %u checkpoints and a %u-step record of %u particles: %s", who, nseg, C,
+                                     n, hipGetErrorString(he)));
+  // ---- forward pass on h (exactly soup_step(h, K)), a checkpoint before every stretch of C steps ----
+  std::vector<uint8_t> swept(K, 0);
+  for (uint32_t k = 0; k < nseg; ++k) {
+    seg_save(h, ck[k]);
+    r = seg_run(h, k * C, std::min(K, (k + 1) * C), K, call_start, swept, nullptr);
+    if (r) return r;
+  }
+  r = sync(h);
+  if (r) return r;
+  HC(h, hipMemsetAsync(lx, 0, 3 * (size_t)n * sizeof(double), h->stream));
+  HC(h, hipMemsetAsync(lv, 0, 3 * (size_t)n * sizeof(double), h->stream));
+  HC(h, hipMemsetAsync(acc, 0, (ADJ_P + 1) * sizeof(double), h->stream));
+  double Jend = 0.0;
+  if (sp->objective == SOUP_OBJ_END_ENERGY) {
+    // J = U(x_K) on the state the pass leaves: dJ/dx_K = g, dJ/dtheta starts at dU/dtheta
+    double pot[5];
+    r = soup_potential(h, pb, pot);
+    if (r) return r;
+    Jend = pot[4];
+    r = assert_lists(h);
+    if (r) return r;
+    adjoint_launch(h, pb, es, nullptr, AdjStrain{{0.0, 0.0, 0.0}});
+    HC(h, hipMemcpyAsync(lx, h->adj_g, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    hipLaunchKernelGGL(k_seg_acc, dim3(1), dim3(64), 0, h->stream, h->adj_out, acc, 1);
+  }
+  r = sync(h);
+  if (r) return r;
+  // ---- reverse sweep on a replay system: each stretch recomputed from its checkpoint ----
+  SoupHandle* rp = nullptr;
+  r = seg_clone(h, &rp);
+  if (r) return r;
+  std::unique_ptr<SoupHandle, int (*)(SoupHandle*)> guard(rp, soup_destroy);
+  bool es_r = false;
+  r = adjoint_ready(rp, who, pb, es_r);
+  if (r) return fail(h, r, rp->err);
+  const double dt = c.dt, a = 1.0 - (double)c.gamma * (double)c.dt;
+  AdjStrain obj{{0.0, 0.0, 0.0}};
+  if (sp->objective == SOUP_OBJ_MEAN_STRAIN_VIRIAL)
+    for (int q = 0; q < 3; ++q) obj.e[q] = sp->strain[q] * sp->scale / (double)K;
+  const AdjStrain none{{0.0, 0.0, 0.0}};
+  const uint32_t n3 = 3u * n;
+  // one launch at a recorded state: positions of rec[ip], topology of rec[it], direction cw * lv
+  // (cw = 0: none) plus the strain st
+  auto vjp = [&](const StateRefs& P, const StateRefs& T, double cw, const AdjStrain& st) {
+    StateRefs ld{};
+    ld.pos = P.pos;
+    ld.charge = T.charge;
+    ld.bs = T.bs;
+    ld.clink = T.clink;
+    hipLaunchKernelGGL(k_state_scatter, dim3(blocks(n, 256)), dim3(256), 0, rp->stream, n, rp->b, ld);
+    ph_rebuild(rp);
+    if (cw != 0.0) hipLaunchKernelGGL(k_seg_w, dim3(blocks(n3, 256)), dim3(256), 0, rp->stream, n3, lv, cw, rp->adj_w);
+    adjoint_launch(rp, pb, es_r, cw != 0.0 ? (const double*)rp->adj_w : nullptr, st);
+    hipLaunchKernelGGL(k_seg_lx, dim3(blocks(n, 256)), dim3(256), 0, rp->stream, n, lx, rp->adj_hw, rp->adj_g, st);
+    hipLaunchKernelGGL(k_seg_acc, dim3(1), dim3(64), 0, rp->stream, rp->adj_out, acc, 0);
+  };
+  for (int k = (int)nseg - 1; k >= 0; --k) {
+    const uint32_t s0 = (uint32_t)k * C, s1 = std::min(K, s0 + C);
+    seg_restore(rp, ck[k]);
+    {
+      const StateRefs t0 = state_part(ck[k].st, true);
+      const StateRefs& d0 = rec[0];
+      HC(h, hipMemcpyAsync(d0.pos, t0.pos, 16 * (size_t)n, hipMemcpyDeviceToDevice, rp->stream));
+      HC(h, hipMemcpyAsync(d0.charge, t0.charge, 4 * (size_t)n, hipMemcpyDeviceToDevice, rp->stream));
+      HC(h, hipMemcpyAsync(d0.bs, t0.bs, 16 * (size_t)n, hipMemcpyDeviceToDevice, rp->stream));
+      HC(h, hipMemcpyAsync(d0.clink, t0.clink, 4 * (size_t)n, hipMemcpyDeviceToDevice, rp->stream));
+    }
+    r = seg_run(rp, s0, s1, K, call_start, swept, &rec);
+    if (r) return fail(h, r, rp->err);
+    for (uint32_t s = s1; s-- > s0;) {
+      const uint32_t i = s - s0 + 1u;
+      const bool bond_step = (call_start + s) % c.bond_interval == 0;
+      const bool has_obj = sp->objective == SOUP_OBJ_MEAN_STRAIN_VIRIAL;
+      const bool last = s == K - 1u;
+      // x_{s+1} is where step s's second half-kick (a dt/2, topology at the start of step s) and step
+      // s + 1's first half-kick (dt/2; after a sweep, with the post-sweep topology) take their force,
+      // and where the objective observes the state after step s
+      double cw_start = a * dt / 2.0 + (last ? 0.0 : dt / 2.0);
+      bool obj_done = !has_obj;
+      if (!last && swept[s]) {
+        vjp(rec[i], rec[i], dt / 2.0, has_obj ? obj : none);
+        cw_start = a * dt / 2.0;
+        obj_done = true;
+      }
+      if (!obj_done && bond_step) {  // the bonds may have changed during step s
+        vjp(rec[i], rec[i], 0.0, obj);
+        obj_done = true;
+      }
+      vjp(rec[i], rec[i - 1u], cw_start, obj_done ? none : obj);
+      hipLaunchKernelGGL(k_seg_lv, dim3(blocks(n3, 256)), dim3(256), 0, rp->stream, n3, lv, lx, a, dt);
+    }
+    if (k == 0) vjp(rec[0], rec[0], dt / 2.0, none);  // step 0's first half-kick at x_0
+    r = sync(rp);
+    if (r) return fail(h, r, rp->err);
+    r = assert_lists(rp);
+    if (r) return fail(h, r, rp->err);
+  }
+  double accv[ADJ_P + 1];
+  HC(h, hipMemcpy(accv, acc, sizeof(accv), hipMemcpyDeviceToHost));
+  for (uint32_t p = 0; p < ADJ_P; ++p) dJ_dtheta[p] = accv[p];
+  *J = sp->objective == SOUP_OBJ_END_ENERGY ? Jend : accv[ADJ_P];
+  if (dJ_dx0) HC(h, hipMemcpy(dJ_dx0, lx, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  if (dJ_dv0) HC(h, hipMemcpy(dJ_dv0, lv, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

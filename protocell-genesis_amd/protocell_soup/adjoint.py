@@ -12,6 +12,14 @@ written to the device (rounded to f32, the precision the system holds), the ener
 force is evaluated there, and backward is one adjoint launch. SoupForce's backward is the
 vector-Jacobian product of the force, -H w for the positions and d(w . F)/dtheta for theta, so a
 loss on forces (force matching, a virial) differentiates through it.
+
+segmentGradient differentiates a trajectory observable over a segment of steps (soup_segment_adjoint:
+device checkpoints every checkpointEvery steps, each stretch recomputed forward before its reverse
+sweep). Its observables are the end energy and the window mean of a strain virial; the lateral
+tension of a membrane normal to z is the one the area-per-lipid sensitivity goes through: at fixed
+box the area per lipid a cannot move, but the tension gamma(a, theta) it holds can, and at the
+tensionless state a barostat would find, da/dtheta = -(a / K_A) dgamma/dtheta with K_A = a dgamma/da
+the area compressibility modulus (areaPerLipidSensitivity, areaCompressibility).
 """
 from __future__ import annotations
 
@@ -62,29 +70,72 @@ def basisWith(b: native.SoupPotentialBasis, theta: Sequence[float], headCharge: 
     return out
 
 
-def energyAdjoint(sys_, basis: Optional[native.SoupPotentialBasis] = None, w: Optional[np.ndarray] = None) -> Dict[str, Any]:
+def energyAdjoint(sys_, basis: Optional[native.SoupPotentialBasis] = None, w: Optional[np.ndarray] = None,
+                  strain: Optional[Sequence[float]] = None) -> Dict[str, Any]:
     """One adjoint launch at the system's configuration: grad = dU/dx (N, 3), dU = dU/dtheta, and with
-    a direction w (N, 3): hw = H w and dFw = d(w . F)/dtheta. FP64 throughout."""
+    a direction w (N, 3): hw = H w and dFw = d(w . F)/dtheta. FP64 throughout.
+
+    With a diagonal strain e (3 values) every pair term's direction also takes diag(e) r_ij: then
+    We = W_e, the strain virial (sum over pair terms of (diag(e) r_ij) . f_ij), hw = H w + H_e and
+    dFw = d(w . F)/dtheta + dW_e/dtheta, and dW_e/dx = -diag(e) grad - H_e (virialGradient)."""
     b = basis if basis is not None else sys_.potentialBasis()
     n = sys_.N
     grad = np.empty(3 * n, dtype=np.float64)
     dU = np.empty(14, dtype=np.float64)
     hw = dFw = None
     wp = hp = fp = None
+    if w is not None or strain is not None:
+        hw = np.empty(3 * n, dtype=np.float64)
+        dFw = np.empty(14, dtype=np.float64)
+        hp, fp = hw.ctypes.data_as(C.c_void_p), dFw.ctypes.data_as(C.c_void_p)
     if w is not None:
         wa = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
         if wa.size != 3 * n:
             raise ValueError(f"energyAdjoint: w holds {wa.size // 3} particles, the system {n}")
-        hw = np.empty(3 * n, dtype=np.float64)
-        dFw = np.empty(14, dtype=np.float64)
-        wp, hp, fp = wa.ctypes.data_as(C.c_void_p), hw.ctypes.data_as(C.c_void_p), dFw.ctypes.data_as(C.c_void_p)
-    sys_._call(sys_._lib.soup_energy_adjoint, C.byref(b), wp, grad.ctypes.data_as(C.c_void_p), hp,
-               dU.ctypes.data_as(C.c_void_p), fp)
+        wp = wa.ctypes.data_as(C.c_void_p)
     out = {"grad": grad.reshape(n, 3), "dU": dU}
-    if w is not None:
+    if strain is None:
+        sys_._call(sys_._lib.soup_energy_adjoint, C.byref(b), wp, grad.ctypes.data_as(C.c_void_p), hp,
+                   dU.ctypes.data_as(C.c_void_p), fp)
+    else:
+        e = (C.c_double * 3)(*[float(x) for x in strain])
+        We = C.c_double(0.0)
+        sys_._call(sys_._lib.soup_energy_adjoint_strain, C.byref(b), e, wp, grad.ctypes.data_as(C.c_void_p), hp,
+                   dU.ctypes.data_as(C.c_void_p), fp, C.byref(We))
+        out["We"] = We.value
+    if hw is not None:
         out["hw"] = hw.reshape(n, 3)
         out["dFw"] = dFw
     return out
+
+
+def virialGradient(sys_, strain: Sequence[float], basis: Optional[native.SoupPotentialBasis] = None) -> Dict[str, Any]:
+    """The strain virial W_e of the current configuration and its gradients: dx = dW_e/dx (N, 3) and
+    dtheta = dW_e/dtheta."""
+    r = energyAdjoint(sys_, basis, strain=strain)
+    e = np.asarray(strain, dtype=np.float64)
+    return {"W": r["We"], "dx": -r["grad"] * e - r["hw"], "dtheta": r["dFw"]}
+
+
+def tensionStrain(sys_) -> tuple:
+    """The strain and scale whose W_e is the configurational lateral tension of a membrane normal to
+    z, (W_zz - (W_xx + W_yy) / 2) / (Lx Ly)."""
+    L = sys_.box
+    return (-0.5, -0.5, 1.0), 1.0 / (float(L[0]) * float(L[1]))
+
+
+def areaCompressibility(areas: Sequence[float], kT: float) -> float:
+    """K_A = kT <A> / var(A) from a trace of the projected area A sampled at zero tension (the area
+    move's accepted boxes)."""
+    a = np.asarray(areas, dtype=np.float64)
+    return float(kT * a.mean() / a.var())
+
+
+def areaPerLipidSensitivity(dtension_dtheta: np.ndarray, area_per_lipid: float, KA: float) -> np.ndarray:
+    """da/dtheta = -(a / K_A) dgamma/dtheta: how far the tensionless area per lipid a moves per unit
+    of each parameter, from the fixed-area tension sensitivity (implicit function theorem on
+    gamma(a*(theta), theta) = 0)."""
+    return -(float(area_per_lipid) / float(KA)) * np.asarray(dtension_dtheta, dtype=np.float64)
 
 
 def setPositions(sys_, x: np.ndarray) -> None:
@@ -141,98 +192,48 @@ def __getattr__(name):
 
 
 # ---- trajectory segments -------------------------------------------------------------------------
-def _topology(sys_, pos: np.ndarray, bonds: np.ndarray, links: np.ndarray, charges: np.ndarray) -> None:
-    """Loads one recorded state into a system and rebuilds its lists (positions, bond slots,
-    centre links, charges)."""
-    sys_._write("POS", np.ascontiguousarray(pos, dtype=np.float32).reshape(-1))
-    sys_._write("BOND_SLOTS", np.ascontiguousarray(bonds, dtype=np.uint32).reshape(-1))
-    sys_._write("CENTER_LINK", np.ascontiguousarray(links, dtype=np.uint32).reshape(-1))
-    sys_._write("CHARGE", np.ascontiguousarray(charges, dtype=np.float32).reshape(-1))
-    sys_._call(sys_._lib.soup_forces, native.FORCES_REBUILD)
-
-
-def _clone(sys_):
-    """A second handle holding the same state and parameters (for the reverse sweep's launches)."""
-    from .checkpoint import decodeCheckpointResume, encodeCheckpoint
-    from .sim import createSoup
-
-    R = sys_._R
-    opts = dict(R.opts)
-    opts["resume"] = decodeCheckpointResume(encodeCheckpoint(sys_, {}))
-    c = createSoup(opts)
-    c._call(c._lib.soup_set_params, C.byref(R.cfg))
-    return c
-
-
-def segmentGradient(sys_, steps: int, basis: Optional[native.SoupPotentialBasis] = None) -> Dict[str, Any]:
+def segmentGradient(sys_, steps: int, basis: Optional[native.SoupPotentialBasis] = None, checkpointEvery: int = 200,
+                    objective: str = "energy", strain: Optional[Sequence[float]] = None, scale: float = 1.0) -> Dict[str, Any]:
     """Advances the system by `steps` steps, exactly as step(steps) would, and returns the gradient of
-    J = U(x_K; theta), the potential energy at the end, with respect to theta, to the start positions
-    x_0 and to the start velocities v_0, by one reverse sweep over the recorded states.
+    a trajectory observable J with respect to theta, to the start positions x_0 and to the start
+    velocities v_0, from one reverse sweep (soup_segment_adjoint):
+    - objective "energy": J = U(x_K; theta), the potential energy at the end;
+    - "tension": J = the mean over the states after each step of the configurational lateral tension
+      (tensionStrain), the observable of the area-per-lipid sensitivity;
+    - "virial": J = scale / K * sum over those states of W_e for the given strain.
 
     The step is the reference's velocity-Verlet Langevin step (step.wgsl:96-142): vh = v + dt/2 F(x),
     x' = x + dt vh, v' = (1 - gamma dt)(vh + dt/2 F(x')) + noise. The noise does not depend on the
-    state, so the pathwise adjoint runs backward through the two half-kicks and the drift, with one
-    adjoint launch (-H w and d(w . F)/dtheta) per force evaluation. Bond and charge Monte Carlo are
-    not differentiated: each force is taken with the bond slots, centre links and charges it was
-    computed with, as recorded; a constant-pH sweep inside the segment splits the force a step
-    starts from from the one the previous step ended with, and both are honoured. The reverse
-    launches run on a second handle, so the system continues exactly as step(steps) leaves it.
-    The start must not follow a bond Monte Carlo step (the force buffer would hold the previous
-    topology's force)."""
-    R = sys_._R
-    cfg = R.cfg
-    n = sys_.N
-    g0 = sys_.steps
-    if g0 > 0 and (g0 - 1) % cfg.bond_interval == 0:
-        raise ValueError("segmentGradient: the segment starts right after a bond Monte Carlo step; start one step later")
-    if int((sys_.frozen() != 0).sum()) > 0:
-        raise ValueError("segmentGradient: frozen particles are not differentiated")
+    state, so the pathwise adjoint runs backward through the two half-kicks and the drift with one
+    force vector-Jacobian product per force evaluation, each with the bond slots, centre links and
+    charges that force was computed with; Monte Carlo moves are not differentiated. The library
+    keeps a checkpoint of the full state on the device every `checkpointEvery` steps and recomputes
+    each stretch from it on a replay system before sweeping it backwards, so the device holds one
+    stretch of states, not the whole segment. The start step must be a multiple of rebuildEvery and
+    must not follow a bond Monte Carlo step; frozen particles are not differentiated."""
     b = basis if basis is not None else sys_.potentialBasis()
-    K = int(steps)
-    x0 = sys_.particles().reshape(n, 4).copy()
-    B0, L0, C0 = sys_.bondSlots().reshape(n, 3).copy(), sys_.centerLinks().copy(), sys_.charges().copy()
-    P = np.empty((K, n, 4), np.float32)
-    Bs = np.empty((K, n, 3), np.uint32)
-    Ls = np.empty((K, n), np.uint32)
-    Cs = np.empty((K, n), np.float32)
-    sw = np.zeros(K, np.uint8)
-    sys_._call(sys_._lib.soup_step_record, C.c_uint32(K), P.ctypes.data_as(C.c_void_p), Bs.ctypes.data_as(C.c_void_p),
-               Ls.ctypes.data_as(C.c_void_p), Cs.ctypes.data_as(C.c_void_p), sw.ctypes.data_as(C.c_void_p))
-    J = sys_.potential(b)["total"]
-    end = energyAdjoint(sys_, b)
-    dt, a = float(cfg.dt), 1.0 - float(cfg.gamma) * float(cfg.dt)
-    clone = _clone(sys_)
-    try:
-        def start_of(s):  # topology at the start of step s
-            return (B0, L0, C0) if s == 0 else (Bs[s - 1], Ls[s - 1], Cs[s - 1])
-
-        def vjp(pos, topo, w):
-            _topology(clone, pos, *topo)
-            r = energyAdjoint(clone, b, w=w)
-            return -r["hw"], r["dFw"]
-
-        lx = end["grad"].copy()
-        lv = np.zeros((n, 3))
-        lth = end["dU"].copy()
-        lvh_next = np.zeros((n, 3))
-        for s in range(K - 1, -1, -1):
-            wA = a * (dt / 2) * lv                       # step s's second half-kick
-            wB = (dt / 2) * lvh_next if s < K - 1 else None  # step s + 1's first half-kick
-            if wB is not None and sw[s]:
-                gx, gt = vjp(P[s], (Bs[s], Ls[s], Cs[s]), wB)
-                lx += gx
-                lth += gt
-                wB = None
-            w = wA if wB is None else wA + wB
-            gx, gt = vjp(P[s], start_of(s), w)
-            lx += gx
-            lth += gt
-            lvh = a * lv + dt * lx
-            lv = lvh
-            lvh_next = lvh
-        gx, gt = vjp(x0, (B0, L0, C0), (dt / 2) * lvh_next)  # step 0's first half-kick
-        lx += gx
-        lth += gt
-    finally:
-        clone.dispose()
-    return {"J": J, "dJ_dtheta": lth, "dJ_dx0": lx, "dJ_dv0": lv, "steps": K, "sweeps": int(sw.sum())}
+    spec = native.SoupSegmentSpec()
+    spec.steps = int(steps)
+    spec.checkpoint_every = int(checkpointEvery)
+    if objective == "energy":
+        spec.objective = native.OBJ_END_ENERGY
+    elif objective in ("tension", "virial"):
+        spec.objective = native.OBJ_MEAN_STRAIN_VIRIAL
+        if objective == "tension":
+            strain, scale = tensionStrain(sys_)
+        elif strain is None:
+            raise ValueError("segmentGradient: the virial objective needs a strain")
+        for a in range(3):
+            spec.strain[a] = float(strain[a])
+        spec.scale = float(scale)
+    else:
+        raise ValueError(f"segmentGradient: objective {objective!r} is 'energy', 'tension' or 'virial'")
+    n = sys_.N
+    J = C.c_double(0.0)
+    dth = np.empty(14, dtype=np.float64)
+    dx0 = np.empty(3 * n, dtype=np.float64)
+    dv0 = np.empty(3 * n, dtype=np.float64)
+    sys_._call(sys_._lib.soup_segment_adjoint, C.byref(b), C.byref(spec), C.byref(J), dth.ctypes.data_as(C.c_void_p),
+               dx0.ctypes.data_as(C.c_void_p), dv0.ctypes.data_as(C.c_void_p))
+    return {"J": J.value, "dJ_dtheta": dth, "dJ_dx0": dx0.reshape(n, 3), "dJ_dv0": dv0.reshape(n, 3), "steps": int(steps),
+            "objective": objective}

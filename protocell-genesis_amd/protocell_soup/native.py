@@ -85,6 +85,14 @@ class SoupPotentialBasis(C.Structure):
                 ("use_center_link", C.c_uint32), ("use_charges", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
+class SoupSegmentSpec(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("checkpoint_every", C.c_uint32), ("objective", C.c_uint32), ("reserved", C.c_uint32),
+                ("strain", C.c_double * 3), ("scale", C.c_double)]
+
+
+OBJ_END_ENERGY, OBJ_MEAN_STRAIN_VIRIAL = 0, 1  # soup_segment_adjoint objectives
+
+
 class SoupWaterCounts(C.Structure):
     _fields_ = [("total_empty_cells", C.c_uint64), ("unreached_cells", C.c_uint64), ("reached_empty_cells", C.c_uint64),
                 ("total_water", C.c_uint32), ("encapsulated", C.c_uint32), ("reached_empty_water", C.c_uint32),
@@ -101,7 +109,7 @@ NOALLOC_MAX_N, NOALLOC_MAX_CELLS = 65536, 262144  # soup_potential_at_noalloc li
 EXPORTS = (
     "soup_create", "soup_destroy", "soup_last_error", "soup_step", "soup_step_batch", "soup_step_record", "soup_set_params", "soup_reset_particles", "soup_sample_insertion", "soup_force_candidate_stats", "soup_step_phases", "soup_global_step", "soup_read", "soup_write",
     "soup_field_bytes", "soup_forces", "soup_run_phases", "soup_protonation_sweep", "soup_protonation_rng",
-    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_potential_at_noalloc", "soup_energy_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
+    "soup_last_sweep", "soup_nonfinite", "soup_check_lists", "soup_list_stats", "soup_relax", "soup_potential", "soup_potential_at", "soup_potential_at_noalloc", "soup_energy_adjoint", "soup_energy_adjoint_strain", "soup_segment_adjoint", "soup_set_box", "soup_scale_molecules_rigid", "soup_proximity_pairs", "soup_analysis_create", "soup_analysis_destroy", "soup_analysis_last_error",
     "soup_analysis_load", "soup_analysis_load_system", "soup_analysis_amphiphiles", "soup_analysis_amphiphiles_read",
     "soup_analysis_clusters", "soup_analysis_water", "soup_analysis_cavity", "soup_profile", "soup_profile_read", "soup_use_graphs", "soup_version",
 )
@@ -156,6 +164,10 @@ def load_library(path: Optional[str] = None) -> C.CDLL:
         "soup_set_box": (C.c_int, [H, C.POINTER(SoupConfig), C.c_void_p, C.c_uint32]),
         "soup_energy_adjoint": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
+        "soup_energy_adjoint_strain": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+        "soup_segment_adjoint": (C.c_int, [H, C.POINTER(SoupPotentialBasis), C.POINTER(SoupSegmentSpec), C.POINTER(C.c_double),
+                                           C.c_void_p, C.c_void_p, C.c_void_p]),
         "soup_proximity_pairs": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double), C.c_double, C.c_void_p, C.c_void_p,
                                            C.c_uint64, C.POINTER(C.c_uint64)]),
         "soup_scale_molecules_rigid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(C.c_double),

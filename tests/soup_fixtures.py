@@ -54,3 +54,82 @@ def assert_same_state(a, b, what):
     sa, sb = state_of(a), state_of(b)
     for k in sa:
         assert np.array_equal(sa[k], sb[k]), f"{what}: {k} differs"
+
+
+def bilayer_patch_opts(n_lipids=2087, tail=4, L=36.84, seed=5, pH=7.0, ionic=0.01, density=0.8):
+    """A pre-assembled bilayer patch of ``n_lipids`` amphiphiles (head + ``tail`` carbons) normal to z
+    in a cubic box of edge L, with water outside the slab bringing the whole box to ``density``: the
+    construction of tests/water-bilayer-area-move.test.ts:75-194 (two leaflets on a jittered square
+    lattice, tails stacked at the bond lengths, the head outermost, water on a jittered lattice
+    skipping the slab, zero velocities, bondRng i + 1, thermoRng i + 1000003) with longer tails and
+    charged heads. The DH list is sized for the heads' slab (densityVolumeSigma3 = L^2 x the slab
+    thickness)."""
+    import math
+
+    from protocell_soup.init_state import Mulberry32, NONE_U32, link_chain
+    from protocell_soup.rules import loadParams, loadSoup
+
+    soup, p = loadSoup(), loadParams()
+    mons = soup["monomers"]
+    carbon = next(k for k, m in enumerate(mons) if m["kind"] == "carbon")
+    head = next(k for k, m in enumerate(mons) if m["kind"] == "head")
+    water = next(k for k, m in enumerate(mons) if m["id"] == soup["solvent"]["waterId"])
+    b_tt = p["sigma"] * mons[carbon]["radiusSigma"]
+    b_ht = p["sigma"] * (mons[head]["radiusSigma"] + mons[carbon]["radiusSigma"]) / 2
+    b_wt = p["sigma"] * (mons[water]["radiusSigma"] + mons[carbon]["radiusSigma"]) / 2
+    z0, gap = L / 2, b_tt / 2
+    half = gap + (tail - 1) * b_tt + b_ht
+    per = math.ceil(n_lipids / 2)
+    n_side = max(1, math.ceil(math.sqrt(per)))
+    s_l = L / n_side
+    jit = 0.15
+    rng = Mulberry32(seed)
+    r = lambda: float(rng.take(1)[0])  # noqa: E731
+    n_water = round(density * L ** 3) - n_lipids * (1 + tail)
+    cap = n_lipids * (1 + tail) + n_water
+    pos = np.zeros((cap, 4), np.float32)
+    bs = np.full(3 * cap, NONE_U32, np.uint32)
+    k = 0
+    for i in range(n_lipids):
+        leaf = 1 if i < per else -1
+        idx = i if i < per else i - per
+        a, bb = idx % n_side, idx // n_side
+        x = ((a + 0.5) * s_l + (r() * 2 - 1) * jit * s_l + L) % L
+        y = ((bb + 0.5) * s_l + (r() * 2 - 1) * jit * s_l + L) % L
+        zt = [z0 + leaf * (gap + (tail - 1 - m) * b_tt) for m in range(tail)]  # t1 (outer) .. tk (inner)
+        zh = zt[0] + leaf * b_ht
+        pos[k] = [x, y, zh, head]
+        for m in range(tail):
+            pos[k + 1 + m] = [x, y, zt[m], carbon]
+        link_chain(bs, k, list(range(k + 1, k + 1 + tail)))
+        k += 1 + tail
+    z_lo, z_hi = z0 - half - 1.5 * b_wt, z0 + half + 1.5 * b_wt
+    # the lattice fine enough that the planes outside the slab hold every water bead
+    wn = max(1, math.ceil((n_water * L / (L - (z_hi - z_lo))) ** (1 / 3)))
+    sw = L / wn
+    placed = 0
+    for iz in range(wn):
+        z_raw = (iz + 0.5) * sw
+        if z_lo < z_raw < z_hi:
+            continue
+        for iy in range(wn):
+            for ix in range(wn):
+                if k >= cap:
+                    break
+                x = ((ix + 0.5) * sw + (r() * 2 - 1) * jit * sw + L) % L
+                y = ((iy + 0.5) * sw + (r() * 2 - 1) * jit * sw + L) % L
+                z = min(L - 1e-4, max(0.0, z_raw + (r() * 2 - 1) * jit * sw))
+                pos[k] = [x, y, z, water]
+                k += 1
+                placed += 1
+    n = k
+    start = {m["id"]: 0 for m in mons}
+    start[mons[head]["id"]], start[mons[carbon]["id"]], start[mons[water]["id"]] = n_lipids, tail * n_lipids, placed
+    resume = {"globalStep": 0, "liveBox": [L, L, L], "positions": pos[:n].reshape(-1),
+              "velocities": np.zeros(4 * n, np.float32), "bondSlots": bs[: 3 * n],
+              "centerLink": np.full(n, NONE_U32, np.uint32), "centerHeldSteps": np.zeros(n, np.uint32),
+              "desorbEvents": np.zeros(2, np.uint32), "bondRng": np.arange(1, n + 1, dtype=np.uint32),
+              "thermoRng": np.arange(1000003, 1000003 + n, dtype=np.uint32), "events": {}}
+    es = {"enabled": True, "pH": pH, "ionicStrengthMolar": ionic, "densityVolumeSigma3": L * L * 2 * (half + 1.0)}
+    return {"box": [L, L, L], "seed": seed, "kT": p["thermostat"]["kT"], "clay": False, "start": start, "resume": resume,
+            "electrostatics": es}

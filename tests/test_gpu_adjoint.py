@@ -167,7 +167,7 @@ def test_torch_energy_and_force_functions(soup):
 def _segment_soup():
     s = ps.createSoup(amphiphile_opts(16.0, 80, 4, {"W": 1800}, seed=6, electrostatics=CHARGED))
     s.relaxColdStart()
-    s.step(65)  # not right after a bond Monte Carlo step (every 20)
+    s.step(70)  # a multiple of rebuildEvery, not right after a bond Monte Carlo step (every 20)
     return s
 
 
@@ -182,7 +182,7 @@ def test_segment_gradient_against_perturbed_reruns():
     try:
         b = s.potentialBasis()
         res = segmentGradient(s, K, b)
-        assert s.steps == 65 + K
+        assert s.steps == 70 + K
         bonds_end = s.bondSlots()
         pos_end = s.particles()
     finally:
