@@ -53,3 +53,24 @@ def build_library(verbose: bool = False, out: str = LIB, defines: dict = None) -
         print(res.stderr[-2000:])
     os.replace(tmp, out)
     return out
+
+
+NAPI_SRC = os.path.join(PKG_ROOT, "napi", "soup_napi.cc")
+NAPI_OUT = os.path.join(LIB_DIR, "soup_napi.node")
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi(out: str = NAPI_OUT) -> str:
+    """g++ of napi/soup_napi.cc against Node's node_api.h, linked to libsoup_hip.so beside it (rpath
+    $ORIGIN). Returns the path, or "" where Node's headers are not installed."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        return ""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = out + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I", NODE_INCLUDE, "-I",
+           os.path.join(PKG_ROOT, "..", "include"), NAPI_SRC, "-L", LIB_DIR, "-lsoup_hip", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ of the N-API addon failed:\n" + res.stderr[-4000:])
+    os.replace(tmp, out)
+    return out
