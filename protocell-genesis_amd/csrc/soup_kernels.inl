@@ -451,7 +451,9 @@ __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t c
 #ifndef VL_STAGE
 #define VL_STAGE 4608u  // stencil beads one workgroup of k_verlet_cells stages in LDS (16 B each)
 #endif
-#define VL_WAVES 16u    // wavefronts per workgroup of k_verlet_cells (two workgroups fill a CU)
+#ifndef VL_WAVES
+#define VL_WAVES 16u  // wavefronts per workgroup of k_verlet_cells (two workgroups fill a CU)
+#endif
 #define VL_PAD (VL_BATCH * WAVE)  // staged padding the sweep may read past the stencil
 static_assert(VL_STAGE >= 2u * VL_WAVES * WAVE + VL_WAVES * WAVE / 4u, "the gather path's tables alias the staging");
 

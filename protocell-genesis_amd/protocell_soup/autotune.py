@@ -7,11 +7,21 @@ streams are untouched, and the force sums stay in FP64 rounded once):
     FORCE_TPP    lanes per particle in the list force (4, 8 or 16)
     GROUP_BATCH  list entries per lane whose loads the list force issues together
     FAR_BATCH    far Debye-Hueckel chunks of 64 loaded together
-    VL_BATCH     Verlet-build candidate chunks of 64 loaded together
+    VL_BATCH     Verlet-build candidate chunks of 64 swept together
     ES_BATCH     DH-list-build candidate chunks of 64 loaded together
+    VL_CELLS     the Verlet build as one workgroup per cell with the cell's neighbourhood staged in
+                 LDS (1) or as one wavefront per particle gathering from HBM (0)
+    VL_WAVES     wavefronts per workgroup of the per-cell Verlet build (8 or 16)
 
-The rebuild cadence is not tuned: the Verlet list feeds the bond Monte Carlo candidate walk, so
-rebuildEvery is part of the model (data/soup.json verletList) and the MC decisions follow it.
+The reference's rebuild cadence is not tuned: the Verlet list feeds the bond Monte Carlo candidate
+walk, so rebuildEvery is part of the model (data/soup.json verletList) and the MC decisions follow
+it. The displacement-triggered schedule (verletOverride.rebuild = "drift"), where the lists follow
+the drift instead, has one free parameter, how often the drift maximum is read back
+(driftCheckEvery; its trigger follows from it, skin/2 less the displacement that many steps can
+add). `schedule` times the campaign under each candidate and writes the fastest into the tuning
+file's "rebuild" section, which the drift schedule takes as its default:
+
+    python -m protocell_soup.autotune schedule [STEPS] [OUT]   # on the GPU
 
 Two steps, so that compilation stays on a build host and timing on the GPU:
 
@@ -41,8 +51,11 @@ TUNABLES = {
     "FAR_BATCH": ((8, 16, 32), "es_far"),
     "VL_BATCH": ((2, 4, 8), "verlet_build"),
     "ES_BATCH": ((4, 8, 16), "es_build"),
+    "VL_CELLS": ((0, 1), "verlet_build"),
+    "VL_WAVES": ((8, 16), "verlet_build"),
 }
-DEFAULTS = {"FORCE_TPP": 8, "GROUP_BATCH": 4, "FAR_BATCH": 16, "VL_BATCH": 4, "ES_BATCH": 8}
+DEFAULTS = {"FORCE_TPP": 8, "GROUP_BATCH": 4, "FAR_BATCH": 16, "VL_BATCH": 4, "ES_BATCH": 8, "VL_CELLS": 1, "VL_WAVES": 16}
+DRIFT_CHECK_CANDIDATES = (1, 2, 5, 10, 20)
 CAMPAIGN = {"box": [76, 76, 76], "seed": 19, "kT": 1.1, "clay": False,
             "start": {"C": 55988, "O": 18659, "H": 55988, "M": 1452, "W": 351181},
             "electrostatics": {"enabled": True, "pH": 7.0, "ionicStrengthMolar": 0.01}}
@@ -125,19 +138,80 @@ def measure(directory: str, steps: int = 200, out: str = TUNING) -> Dict[str, in
         best[k] = top["defines"][k]
     result = {"workload": "campaign soup 483,268 beads, 76 sigma, pH 7.0, 0.01 M", "steps": steps, "defines": best,
               "measurements": rows}
+    if os.path.exists(out):  # keep the schedule's section
+        with open(out) as fh:
+            prev = json.load(fh)
+        if "rebuild" in prev:
+            result["rebuild"] = prev["rebuild"]
     with open(out, "w") as fh:
         json.dump(result, fh, indent=1)
     print(f"[autotune] chose {best} -> {out}", flush=True)
     return best
 
 
+def _time_schedule(opts: Dict, steps: int) -> Dict[str, float]:
+    """steps/s and rebuilds of the campaign under `opts` over `steps` steps (graphs on), after the
+    cold-start relax and 200 steps."""
+    import torch
+
+    import protocell_soup as ps
+
+    s = ps.createSoup(opts)
+    try:
+        s.relaxColdStart()
+        s.step(200)
+        r0 = s.rebuildCount()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        s.step(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        return {"steps_per_s": steps / dt, "rebuilds": s.rebuildCount() - r0}
+    finally:
+        s.dispose()
+
+
+def schedule(steps: int = 1000, out: str = TUNING) -> int:
+    """Times the campaign under the drift schedule at each driftCheckEvery candidate (and the
+    reference's cadence, for the record) and writes the fastest candidate into `out`'s "rebuild"
+    section, keeping its launch shapes."""
+    rows = [dict(_time_schedule(CAMPAIGN, steps), schedule="cadence")]
+    print(f"[autotune] cadence: {rows[0]['steps_per_s']:.1f} steps/s, {rows[0]['rebuilds']} rebuilds", flush=True)
+    for e in DRIFT_CHECK_CANDIDATES:
+        r = _time_schedule(dict(CAMPAIGN, verletOverride={"rebuild": "drift", "driftCheckEvery": e}), steps)
+        rows.append(dict(r, schedule="drift", driftCheckEvery=e))
+        print(f"[autotune] drift, check every {e}: {r['steps_per_s']:.1f} steps/s, {r['rebuilds']} rebuilds", flush=True)
+    best = max((r for r in rows if r["schedule"] == "drift"), key=lambda r: r["steps_per_s"])
+    data = {}
+    if os.path.exists(out):
+        with open(out) as fh:
+            data = json.load(fh)
+    data["rebuild"] = {"workload": "campaign soup 483,268 beads, 76 sigma, pH 7.0, 0.01 M", "steps": steps,
+                       "driftCheckEvery": best["driftCheckEvery"], "measurements": rows}
+    with open(out, "w") as fh:
+        json.dump(data, fh, indent=1)
+    print(f"[autotune] drift schedule: check every {best['driftCheckEvery']} -> {out}", flush=True)
+    return best["driftCheckEvery"]
+
+
+def tuned_drift_check_every(path: str = TUNING, default: int = 5) -> int:
+    """The tuned driftCheckEvery of the drift schedule (`default` when none has been measured)."""
+    if not os.path.exists(path):
+        return default
+    with open(path) as fh:
+        v = json.load(fh).get("rebuild", {}).get("driftCheckEvery")
+    return int(v) if isinstance(v, int) and v in DRIFT_CHECK_CANDIDATES else default
+
+
 def main(argv=None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
-    if not argv or argv[0] not in ("build", "measure", "time"):
+    if not argv or argv[0] not in ("build", "measure", "time", "schedule"):
         print(__doc__)
         return 1
     if argv[0] == "build":
         build(argv[1])
+    elif argv[0] == "schedule":
+        schedule(int(argv[1]) if len(argv) > 1 else 1000, argv[2] if len(argv) > 2 else TUNING)
     elif argv[0] == "measure":
         measure(argv[1], int(argv[2]) if len(argv) > 2 else 200, argv[3] if len(argv) > 3 else TUNING)
     else:

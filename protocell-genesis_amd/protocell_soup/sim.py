@@ -174,21 +174,32 @@ def resolve(opts: Dict[str, Any]) -> Resolved:
 def resolveRebuildSchedule(soup: dict, p: dict, kT: float, vo: Dict[str, Any]) -> Dict[str, Any]:
     """The Verlet rebuild schedule: the reference's fixed cadence (verletList.rebuildEvery,
     soup/src/soup-integrate.ts:149), or, with verletOverride.rebuild = "drift", a rebuild whenever the
-    measured drift maximum exceeds a trigger, checked every driftCheckEvery steps. The default trigger
+    measured drift maximum exceeds a trigger, checked every driftCheckEvery steps (default: the
+    autotuner's choice for the campaign, else 5). The default trigger
     leaves the reference's own per-step displacement estimate, dt * 2 sqrt(3 kT) (the drift condition
     of data/soup.json's verletList basis), for every step until the next check: skin/2 - checkEvery *
-    dt * 2 sqrt(3 kT)."""
+    dt * 2 sqrt(3 kT). The tuned interval was measured at the campaign's temperature and step; where
+    it leaves no margin (a hotter or coarser soup), the largest interval up to it that does is taken.
+    An interval the caller names is kept as given."""
     mode = vo.get("rebuild", "cadence")
     if mode not in ("cadence", "drift"):
         raise ValueError(f"verletOverride.rebuild={mode!r}: the schedule is 'cadence' (the reference's) or 'drift'")
     if mode == "cadence":
         return {"mode": "cadence"}
     skin = soup["verletList"]["skin"]
-    every = int(vo.get("driftCheckEvery", 5))
+    from .autotune import tuned_drift_check_every
+
+    v_bound = 2.0 * math.sqrt(3.0 * kT)
+    step_drift = p["integrator"]["dt"] * v_bound
+    if "driftCheckEvery" in vo:
+        every = int(vo["driftCheckEvery"])
+    else:
+        every = tuned_drift_check_every()
+        while every > 1 and skin / 2 - every * step_drift <= 0.0:
+            every -= 1
     if not 1 <= every <= 1000:
         raise ValueError(f"verletOverride.driftCheckEvery={every} must be 1..1000")
-    v_bound = 2.0 * math.sqrt(3.0 * kT)
-    trigger = float(vo.get("driftTrigger", skin / 2 - every * p["integrator"]["dt"] * v_bound))
+    trigger = float(vo.get("driftTrigger", skin / 2 - every * step_drift))
     if not 0.0 < trigger <= skin / 2:
         raise ValueError(f"verletOverride.driftTrigger={trigger:.4f} must lie in (0, skin/2={skin / 2}] "
                          f"(at driftCheckEvery={every} the default margin leaves none)")
