@@ -14,8 +14,9 @@
 //          frozen, bondRng, thermoRng, charges, events, desorbEvents (typed arrays), globalStep,
 //          nextSweepAt, protonationRng (numbers)}) -> handle (external; destroy() or the GC frees it)
 //   destroy(h)
-//   step(h, n) -> Promise<void>              soup_step on a worker thread (the event loop stays free;
-//                                            calls on one handle must not overlap)
+//   step(h, n) -> Promise<void>              soup_step on a worker thread (the event loop stays free);
+//                                            until it settles, every other call on that handle throws
+//                                            "soup_napi: a step is still running on this system"
 //   stepSync(h, n)
 //   read(h, field, typedArray) -> typedArray soup_read into the caller's array (exact byte size)
 //   write(h, field, typedArray)              soup_write
@@ -39,6 +40,9 @@ namespace {
 
 struct Handle {
   SoupHandle* h = nullptr;
+  // set while an asynchronous step runs on a worker thread; read and written on the JS thread only
+  // (Step sets it, step_complete clears it), so no lock is needed
+  bool busy = false;
 };
 
 napi_value throw_error(napi_env env, const std::string& msg) {
@@ -67,10 +71,16 @@ bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value* ar
   return true;
 }
 
+// the live, idle handle behind v (nullptr with a thrown error otherwise): the library's calls on one
+// system do not overlap, as the reference's host awaits each call before the next
 Handle* handle_of(napi_env env, napi_value v) {
   void* p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p || !static_cast<Handle*>(p)->h) {
     throw_error(env, "soup_napi: not a live soup handle");
+    return nullptr;
+  }
+  if (static_cast<Handle*>(p)->busy) {
+    throw_error(env, "soup_napi: a step is still running on this system; await it before the next call");
     return nullptr;
   }
   return static_cast<Handle*>(p);
@@ -224,6 +234,7 @@ void step_execute(napi_env, void* data) {
 
 void step_complete(napi_env env, napi_status, void* data) {
   StepWork* w = static_cast<StepWork*>(data);
+  w->hd->busy = false;
   if (w->status == 0) {
     napi_value undef;
     napi_get_undefined(env, &undef);
@@ -251,6 +262,7 @@ napi_value Step(napi_env env, napi_callback_info info) {
   StepWork* w = new StepWork();
   w->hd = hd;
   w->n = n;
+  hd->busy = true;
   napi_value promise, name;
   napi_create_promise(env, &w->deferred, &promise);
   napi_create_reference(env, argv[0], 1, &w->keep);

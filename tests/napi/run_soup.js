@@ -10,7 +10,12 @@ async function main () {
   const sys = host.createSoup(host.loadBundle(dir))
   try {
     await sys.relaxColdStart()
-    await sys.step(steps)
+    // a call made while the step runs on its worker thread is refused, and the step still completes
+    const running = sys.step(steps)
+    let refused = ''
+    try { await sys.particles() } catch (e) { refused = e.message }
+    await running
+    if (!/a step is still running on this system/.test(refused)) throw new Error(`an overlapping call was not refused: '${refused}'`)
     const write = (name, a) => fs.writeFileSync(path.join(out, name), Buffer.from(a.buffer, a.byteOffset, a.byteLength))
     write('positions.f32', await sys.particles())
     write('velocities.f32', await sys.velocities())
