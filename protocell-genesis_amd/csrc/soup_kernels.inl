@@ -1819,7 +1819,11 @@ __device__ __forceinline__ double pcg_next_u(uint32_t& s) {
   return (double)v / 4294967296.0;
 }
 
-__global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, uint32_t nh, double2* tab, uint32_t* tab_g) {
+// Per head row h: tabE[h * cap + s] = the unit-charge screened-Coulomb energy of list entry s (the
+// entry's head index is the DH list's own, es[h * cap + s]); the acid-soap well, nonzero only within
+// its short reach, is compacted in list order into tabW / tabWg[h * cap + k], k < wcnt[h].
+__global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, uint32_t nh, double* tabE, double* tabW,
+                                                      uint32_t* tabWg, uint32_t* wcnt) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t h = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE;
   if (h >= nh) return;
@@ -1828,163 +1832,258 @@ __global__ __launch_bounds__(256) void k_proton_pairs(SoupConfig c, DevBuf b, ui
   const double bx = c.box[0], by = c.box[1], bz = c.box[2];
   const double cut2 = c.ph_cutoff * c.ph_cutoff;
   const double reach = c.as_rc + c.as_wc, reach2 = reach * reach;
-  for (uint32_t s = lane; s < cnt; s += WAVE) {
-    const uint32_t g = b.es[(size_t)h * c.es_cap + s];
-    tab_g[(size_t)h * c.es_cap + s] = g;
-    const float4 xj = b.pos[b.head_slot[g]];
-    double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
-    dx = dx - js_round(dx / bx) * bx;
-    dy = dy - js_round(dy / by) * by;
-    dz = dz - js_round(dz / bz) * bz;
-    double r2 = dx * dx + dy * dy + dz * dz;
+  const uint64_t lower = (1ull << lane) - 1ull;
+  uint32_t nw = 0;  // wells so far in this row
+  for (uint32_t s0 = 0; s0 < cnt; s0 += WAVE) {
+    const uint32_t s = s0 + lane;
     double e = 0.0, w = 0.0;
-    if (r2 < cut2 && c.ph_A != 0.0) {
-      double r = sqrt(r2);
-      if (r > 0.0 && r < c.ph_cutoff) {
-        double u1 = (c.ph_A * exp(-c.ph_kappa * r)) / r;
-        e = u1 - c.ph_shiftU + c.ph_shiftF * (r - c.ph_cutoff);
+    uint32_t g = 0;
+    if (s < cnt) {
+      g = b.es[(size_t)h * c.es_cap + s];
+      const float4 xj = b.pos[b.head_slot[g]];
+      double dx = (double)xi.x - (double)xj.x, dy = (double)xi.y - (double)xj.y, dz = (double)xi.z - (double)xj.z;
+      dx = dx - js_round(dx / bx) * bx;
+      dy = dy - js_round(dy / by) * by;
+      dz = dz - js_round(dz / bz) * bz;
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < cut2 && c.ph_A != 0.0) {
+        double r = sqrt(r2);
+        if (r > 0.0 && r < c.ph_cutoff) {
+          double u1 = (c.ph_A * exp(-c.ph_kappa * r)) / r;
+          e = u1 - c.ph_shiftU + c.ph_shiftF * (r - c.ph_cutoff);
+        }
       }
-    }
-    if (c.as_scale > 0.0 && r2 < reach2 && r2 > 0.0) {
-      double r = sqrt(r2);
-      if (r < c.as_rc) w = -c.as_scale * c.as_eps;
-      else if (r > c.as_rc + c.as_wc) w = 0.0;
-      else {
-        double cc = cos((3.141592653589793 * (r - c.as_rc)) / (2.0 * c.as_wc));
-        w = -c.as_scale * c.as_eps * cc * cc;
+      if (c.as_scale > 0.0 && r2 < reach2 && r2 > 0.0) {
+        double r = sqrt(r2);
+        if (r < c.as_rc) w = -c.as_scale * c.as_eps;
+        else if (r > c.as_rc + c.as_wc) w = 0.0;
+        else {
+          double cc = cos((3.141592653589793 * (r - c.as_rc)) / (2.0 * c.as_wc));
+          w = -c.as_scale * c.as_eps * cc * cc;
+        }
       }
+      tabE[(size_t)h * c.es_cap + s] = e;
     }
-    tab[(size_t)h * c.es_cap + s] = make_double2(e, w);
+    // a zero well adds exactly nothing to the chain's sum, so only the nonzero ones are kept
+    const uint64_t mw = __builtin_amdgcn_ballot_w64(w != 0.0);
+    if (w != 0.0) {
+      const uint32_t k = nw + (uint32_t)__popcll(mw & lower);
+      tabW[(size_t)h * c.es_cap + k] = w;
+      tabWg[(size_t)h * c.es_cap + k] = g;
+    }
+    nw += (uint32_t)__popcll(mw);
   }
+  if (lane == 0) wcnt[h] = nw;
 }
 
-#define CHAIN_THREADS 256
-#define CHAIN_REG 8u  // row entries per thread held in registers: rows of up to 2,048 entries are prefetched
+#define CHAIN_WAVES 16u
+#define CHAIN_THREADS (CHAIN_WAVES * WAVE)
+#ifndef CHAIN_WINDOW
+#define CHAIN_WINDOW 16u  // stream positions whose sites one round evaluates (bits 0..30 of a head word)
+#endif
+#define CHAIN_REG 15u  // row entries per lane requested together: rows of up to 960 entries in one round trip
+static_assert(CHAIN_WINDOW >= 1u && CHAIN_WINDOW <= 31u, "a head word holds the window's positions in bits 0..30");
 
-// one head's pair-table row, CHAIN_REG entries per thread (entry tid + k * CHAIN_THREADS)
-struct ChainRow {
-  double2 ew[CHAIN_REG];
-  uint32_t g[CHAIN_REG];
-  uint32_t cnt;
-};
-// the head chosen by the next draw from state st (st is a copy)
-__device__ __forceinline__ uint32_t chain_site(uint32_t st, uint32_t nh) {
-  const double hf = floor(pcg_next_u(st) * (double)nh);
-  return hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
+// exp(-dG / kT) out of line: inlined into the chain's round loop, the exponential's constants are
+// hoisted into vector registers for the whole loop and spilled by the row loads
+__device__ __attribute__((noinline)) double chain_boltzmann(double dG, double kT) { return exp(-dG / kT); }
+
+// OR over the 64 lanes: the same DPP butterfly inside each row of 16 lanes as wave_sum_d_dpp, then
+// the four row results read from lanes 0, 16, 32, 48
+__device__ __forceinline__ uint32_t wave_or_dpp(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+  return (uint32_t)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                    __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
 }
-// requests row h's first `lim` entries (the table's rows are es_cap long; entries past the count are
-// loaded but never used)
-__device__ __forceinline__ void chain_load(const DevBuf& b, const double2* tab, const uint32_t* tab_g, uint32_t cap, uint32_t h,
-                                           uint32_t lim, uint32_t tid, ChainRow& r) {
-  const double2* trow = tab + (size_t)h * cap;
-  const uint32_t* grow = tab_g + (size_t)h * cap;
-#pragma unroll
-  for (uint32_t k = 0; k < CHAIN_REG; ++k) {
-    const uint32_t s = tid + k * CHAIN_THREADS;
-    if (s < lim) {
-      r.ew[k] = trow[s];
-      r.g[k] = grow[s];
+
+// The PCG state after k draws is an affine map of the state: s_k = jm[k] s_0 + ja[k] (mod 2^32),
+// with jm[k] = a^k and ja[k] = c (a^(k-1) + ... + 1) for the step s -> a s + c of pcg_next_u.
+__device__ __forceinline__ void chain_jump(uint32_t k, uint32_t& jm, uint32_t& ja) {
+  jm = 1u;
+  ja = 0u;
+  for (uint32_t i = 0; i < k; ++i) {
+    jm *= 747796405u;
+    ja = ja * 747796405u + 2891336453u;
+  }
+}
+// the round's stream from state st, on wavefront 0: lane l <= CHAIN_WINDOW + 1 writes the state
+// after l draws (from its jump constants jm, ja for l draws), lanes <= CHAIN_WINDOW draw l, and
+// lanes < CHAIN_WINDOW mark their site's head word with bit l
+__device__ __forceinline__ void chain_round_stream(uint32_t st, uint32_t nh, uint32_t lane, uint32_t jm, uint32_t ja,
+                                                   uint32_t* hword, uint32_t* s_site, uint32_t* s_state, double* s_u) {
+  if (lane <= CHAIN_WINDOW + 1u) {
+    uint32_t sr = jm * st + ja;
+    s_state[lane] = sr;
+    if (lane <= CHAIN_WINDOW) {
+      const double u = pcg_next_u(sr);
+      s_u[lane] = u;
+      if (lane < CHAIN_WINDOW) {
+        const double hf = floor(u * (double)nh);
+        const uint32_t site = hf < (double)(nh - 1) ? (uint32_t)hf : nh - 1;
+        s_site[lane] = site;
+        atomicOr(&hword[site], 1u << lane);
+      }
     }
   }
-  r.cnt = b.es_count[h];
 }
 
 // stats: [0] attempts, [1] accepted, [2] deprotonated, [3] dEsCount; dstats: [0] dEsSum
-// The serial chain of protonationSweep (electrostatics.ts:435-496) in one workgroup. Every attempt
-// draws its site, and draws the acceptance number only when dG > 0, so the next attempt's site is
-// one of two heads known before this attempt's sum: both rows are requested while this attempt
-// runs, and the one the decision selects is in registers when the next attempt starts.
-__global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, DevBuf b, uint32_t nh, const double2* tab,
-                                                                const uint32_t* tab_g, uint32_t* rng_io, uint32_t* stats,
-                                                                double* dstats) {
-  extern __shared__ float qv[];  // charge of every head
-  __shared__ double s_part[2][2][CHAIN_THREADS / WAVE];
-  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE, nw = CHAIN_THREADS / WAVE;
-  for (uint32_t g = tid; g < nh; g += CHAIN_THREADS) qv[g] = b.charge[b.head_slot[g]];
+// The serial chain of protonationSweep (electrostatics.ts:435-496) in one workgroup, in rounds.
+// The PCG stream is fixed; only how many numbers an attempt consumes (one, or two when dG > 0)
+// depends on the decisions. A round therefore takes the next CHAIN_WINDOW stream positions, reads
+// each one's draw as a site, and evaluates all those sites' work sums at once (one wavefront per
+// position) with the charges as they stand at the round start. Wavefront 0 then walks the attempts
+// in stream order: an attempt's evaluation is exact unless its row (or its own head) holds a head
+// whose charge an earlier attempt of this round flipped, so the walk commits attempts until the
+// first such one, which the next round evaluates again from the updated charges. The decisions,
+// the RNG consumption and the dEs sum are those of the attempt-by-attempt chain. Two workgroup
+// barriers per round: after the stream set-up (wavefront 0), and after the evaluation.
+// Head word in LDS: bit 31 = charged (chargeDeprotonated), bit r = stream position r of the round
+// draws this head as its site.
+__global__ __launch_bounds__(CHAIN_THREADS) void k_proton_chain(SoupConfig c, DevBuf b, uint32_t nh, const double* tabE,
+                                                                const double* tabW, const uint32_t* tabWg, const uint32_t* wcnt,
+                                                                uint32_t* rng_io, uint32_t* stats, double* dstats) {
+  extern __shared__ uint32_t hword[];
+  __shared__ uint32_t s_site[CHAIN_WINDOW], s_M[CHAIN_WINDOW], s_state[CHAIN_WINDOW + 2u], s_ctl[2];
+  __shared__ double s_u[CHAIN_WINDOW + 1u], s_se[CHAIN_WINDOW], s_sa[CHAIN_WINDOW];
+  const uint32_t tid = threadIdx.x, lane = tid & (WAVE - 1), w = tid / WAVE;
+  for (uint32_t g = tid; g < nh; g += CHAIN_THREADS) hword[g] = b.charge[b.head_slot[g]] != 0.0f ? 0x80000000u : 0u;
   __syncthreads();
-  uint32_t state = *rng_io;
-  const double kT = c.ph_kT, q = c.ph_q;
+  const double kT = c.ph_kT, q = c.ph_q, qq = q * (double)(float)q;
   const double dGintr = kT * 2.302585092994046 * (c.ph_pKa - c.ph_pH);
-  const uint32_t cap = c.es_cap, lim = cap < CHAIN_THREADS * CHAIN_REG ? cap : CHAIN_THREADS * CHAIN_REG;
-  uint32_t accepted = 0, dEsCount = 0;
+  const bool soap = c.as_scale > 0.0;
+  const uint32_t cap = c.es_cap, lim = cap < WAVE * CHAIN_REG ? cap : WAVE * CHAIN_REG;
+  uint32_t accepted = 0, dEsCount = 0;  // wavefront 0's
   double dEsSum = 0.0;
-  ChainRow cur, ra, rb;
-  uint32_t h = 0;
-  if (nh > 0) {
-    h = chain_site(state, nh);
-    chain_load(b, tab, tab_g, cap, h, lim, tid, cur);
-  }
-  for (uint32_t a = 0; a < nh; ++a) {
-    (void)pcg_next_u(state);  // this attempt's site draw (h)
-    uint32_t ha = 0, hb = 0;
-    if (a + 1 < nh) {
-      uint32_t s2 = state;
-      (void)pcg_next_u(s2);
-      ha = chain_site(state, nh);  // next site if this attempt draws no acceptance number
-      hb = chain_site(s2, nh);     // next site if it does
-      chain_load(b, tab, tab_g, cap, ha, lim, tid, ra);
-      chain_load(b, tab, tab_g, cap, hb, lim, tid, rb);
-    }
-    const uint32_t cnt = cur.cnt;
-    double se = 0.0, sa = 0.0;
-#pragma unroll
-    for (uint32_t k = 0; k < CHAIN_REG; ++k) {
-      const uint32_t s = tid + k * CHAIN_THREADS;
-      if (s < cnt && s < lim) {
-        const float qj = qv[cur.g[k]];
-        if (qj != 0.0f) se += (q * (double)qj) * cur.ew[k].x;
-        sa += qj == 0.0f ? cur.ew[k].y : -cur.ew[k].y;
-      }
-    }
-    // rows longer than the register window: the rest straight from the table
-    for (uint32_t s = lim + tid; s < cnt; s += CHAIN_THREADS) {
-      const double2 ew = tab[(size_t)h * cap + s];
-      const float qj = qv[tab_g[(size_t)h * cap + s]];
-      if (qj != 0.0f) se += (q * (double)qj) * ew.x;
-      sa += qj == 0.0f ? ew.y : -ew.y;
-    }
-    se = wave_sum_d_dpp(se);
-    sa = wave_sum_d_dpp(sa);
+  uint32_t jm = 1u, ja = 0u;
+  if (w == 0) {
+    chain_jump(lane <= CHAIN_WINDOW + 1u ? lane : 0u, jm, ja);
     if (lane == 0) {
-      s_part[a & 1u][0][w] = se;
-      s_part[a & 1u][1][w] = sa;
+      s_ctl[0] = 0;
+      s_ctl[1] = *rng_io;
+    }
+    if (nh > 0) chain_round_stream(*rng_io, nh, lane, jm, ja, hword, s_site, s_state, s_u);
+  }
+  __syncthreads();
+  uint32_t done = s_ctl[0];
+  while (done < nh) {
+    // ---- evaluate: wavefront w takes positions w, w + CHAIN_WAVES, ... ----
+    for (uint32_t r = w; r < CHAIN_WINDOW; r += CHAIN_WAVES) {
+      const uint32_t h = uni(s_site[r]), below = (1u << r) - 1u;  // uniform: the row bases in scalar registers
+      const size_t row = (size_t)h * cap;
+      // every lane requests CHAIN_REG entries and one acid-soap well with no branch (indices past
+      // the register window clamped to its last entry; lim >= 1 here, since soup_create refuses
+      // charged heads with a DH capacity of 0), so the loads issue back to back; entries past the
+      // counts are never used
+      double ee[CHAIN_REG];
+      uint32_t gg[CHAIN_REG];
+#pragma unroll
+      for (uint32_t k = 0; k < CHAIN_REG; ++k) {
+        const uint32_t s = lane + k * WAVE < lim ? lane + k * WAVE : lim - 1u;
+        ee[k] = tabE[row + s];
+        gg[k] = b.es[row + s];
+      }
+      const uint32_t sw = lane < lim ? lane : lim - 1u;
+      const double wv = tabW[row + sw];
+      const uint32_t wg = tabWg[row + sw];
+      const uint32_t cnt = uni(b.es_count[h]), nwell = uni(wcnt[h]);
+      double se = 0.0, sa = 0.0;
+      uint32_t m = 0;
+      // an unused entry adds exactly 0 (the sums start at +0 and take no exact -0), which leaves
+      // them as the skipped entry would
+#pragma unroll
+      for (uint32_t k = 0; k < CHAIN_REG; ++k) {
+        const bool ok = lane + k * WAVE < cnt && lane + k * WAVE < lim;
+        const uint32_t wd = hword[ok ? gg[k] : h];
+        se += (ok && (wd >> 31) != 0u) ? qq * ee[k] : 0.0;
+        m |= ok ? (wd & below) : 0u;
+      }
+      for (uint32_t s = lim + lane; s < cnt; s += WAVE) {  // rows longer than the register window
+        const uint32_t wd = hword[b.es[row + s]];
+        if ((wd >> 31) != 0u) se += qq * tabE[row + s];
+        m |= wd & below;
+      }
+      if (soap) {
+        const bool okw = lane < nwell && lane < lim;
+        const uint32_t wdw = hword[okw ? wg : h];
+        sa += okw ? ((wdw >> 31) != 0u ? -wv : wv) : 0.0;
+        for (uint32_t s = (lim < WAVE ? lim : WAVE) + lane; s < nwell; s += WAVE)  // more than 64 wells
+          sa += (hword[tabWg[row + s]] >> 31) != 0u ? -tabW[row + s] : tabW[row + s];
+      }
+      se = wave_sum_d_dpp(se);
+      sa = wave_sum_d_dpp(sa);
+      m = wave_or_dpp(m);
+      if (lane == 0) {
+        s_se[r] = se;
+        s_sa[r] = sa;
+        s_M[r] = m | (hword[h] & below);  // earlier positions on this row's heads, or on this head
+      }
     }
     __syncthreads();
-    double te = 0.0, ta = 0.0;
-    for (uint32_t k = 0; k < nw; ++k) {
-      te += s_part[a & 1u][0][k];
-      ta += s_part[a & 1u][1][k];
-    }
-    double uEs = te + (c.as_scale > 0.0 ? ta : 0.0);
-    bool isCharged = qv[h] != 0.0f;
-    double dG = isCharged ? -(dGintr + uEs) : dGintr + uEs;
-    const bool drew = !(dG <= 0.0);
-    bool acc = !drew;
-    if (drew) acc = pcg_next_u(state) < exp(-dG / kT);
-    if (acc) {
-      qv[h] = isCharged ? 0.0f : (float)q;  // every thread writes the same value to its own view
-      accepted++;
-      if (!isCharged) {
-        dEsSum += uEs / kT;
-        dEsCount++;
+    if (w == 0) {
+      // ---- lane p holds position p: its site, dG, acceptance probability and conflict mask ----
+      const bool live = lane < CHAIN_WINDOW;
+      const uint32_t site = live ? s_site[lane] : 0u;
+      const bool charged = live && (hword[site] >> 31) != 0u;
+      const double uEs = live ? s_se[lane] + (soap ? s_sa[lane] : 0.0) : 0.0;
+      const double dG = charged ? -(dGintr + uEs) : dGintr + uEs;
+      const double pacc = (live && !(dG <= 0.0)) ? chain_boltzmann(dG, kT) : 1.0;
+      const double unext = lane < CHAIN_WINDOW ? s_u[lane + 1u] : 0.0;  // the acceptance draw if p draws
+      const uint32_t M = live ? s_M[lane] : 0u;
+      const uint32_t drewm = (uint32_t)__builtin_amdgcn_ballot_w64(live && !(dG <= 0.0));
+      const uint32_t passm = (uint32_t)__builtin_amdgcn_ballot_w64(live && unext < pacc);
+      const uint32_t chgm = (uint32_t)__builtin_amdgcn_ballot_w64(charged);
+      // ---- commit in stream order up to the first attempt an earlier flip of this round touches:
+      // a scalar walk over the lanes' masks ----
+      uint32_t p = 0, flipped = 0, nd = uni(done);
+      while (nd < nh && p < CHAIN_WINDOW) {
+        const uint32_t Mp = uni((uint32_t)__builtin_amdgcn_readlane((int)M, (int)p));
+        if ((Mp & flipped) != 0u) break;
+        const bool drew = (drewm >> p) & 1u;
+        const bool acc = drew ? ((passm >> p) & 1u) != 0u : true;
+        if (acc) flipped |= 1u << p;
+        ++nd;
+        p += drew ? 2u : 1u;
       }
+      // the flips, their counts and the dEs sum in stream order
+      if (live && ((flipped >> lane) & 1u) != 0u) atomicXor(&hword[site], 0x80000000u);  // lane < 32: no shift wraps
+      accepted += (uint32_t)__popc(flipped);
+      uint32_t dep = flipped & ~chgm;  // protonated -> deprotonated
+      dEsCount += (uint32_t)__popc(dep);
+      while (dep) {
+        const uint32_t k = (uint32_t)__builtin_ctz(dep);
+        dep &= dep - 1u;
+        dEsSum += readlane_d(uEs, (int)k) / kT;
+      }
+      done = nd;
+      const uint32_t st = s_state[p];
+      wave_lds_sync();
+      // ---- the next round's stream: this round's marks off, the next ones on ----
+      if (lane < CHAIN_WINDOW) atomicAnd(&hword[site], ~(1u << lane));
+      wave_lds_sync();
+      if (lane == 0) {
+        s_ctl[0] = nd;
+        s_ctl[1] = st;
+      }
+      if (nd < nh) chain_round_stream(st, nh, lane, jm, ja, hword, s_site, s_state, s_u);
     }
-    if (a + 1 < nh) {
-      h = drew ? hb : ha;
-      cur = drew ? rb : ra;
-    }
+    __syncthreads();
+    done = s_ctl[0];
   }
   __syncthreads();
   uint32_t dep = 0;
   for (uint32_t g = tid; g < nh; g += CHAIN_THREADS) {
-    float v = qv[g];
-    b.charge[b.head_slot[g]] = v;
-    dep += v != 0.0f ? 1u : 0u;
+    const bool on = (hword[g] >> 31) != 0u;
+    b.charge[b.head_slot[g]] = on ? (float)q : 0.0f;
+    dep += on ? 1u : 0u;
   }
   dep = wave_sum_u(dep);
   if (lane == 0) atomicAdd(&stats[2], dep);
   if (tid == 0) {
-    *rng_io = state;
+    *rng_io = s_ctl[1];
     stats[0] = nh;
     stats[1] = accepted;
     stats[3] = dEsCount;

@@ -49,8 +49,9 @@ struct SoupHandle {
   int device = 0;                // the HIP device current at creation, which holds every buffer
   DevBuf b{};
   DevBuf alt{};
-  double2* ptab = nullptr;
-  uint32_t* ptab_g = nullptr;
+  double2* ptab = nullptr;     // constant-pH pair table: per DH entry the energy (first nh x cap), then the wells
+  uint32_t* ptab_g = nullptr;  // the wells' head indices
+  uint32_t* pwcnt = nullptr;   // wells per head row
   uint32_t* pstats = nullptr;  // [0..3] chain stats, [4] rng
   double* pdstats = nullptr;
   unsigned long long* lsum = nullptr;
@@ -475,14 +476,16 @@ static int assert_lists(SoupHandle* h) {
 static int sweep(SoupHandle* h, SoupSweepResult* out) {
   hipEvent_t e_sweep = prof_begin(h, SOUP_PROF_SWEEP);
   ph_rebuild(h);  // DH list at the current positions
+  double* tabE = reinterpret_cast<double*>(h->ptab);
+  double* tabW = tabE + (size_t)h->nh * h->c.es_cap;
   hipLaunchKernelGGL(k_proton_pairs, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh,
-                     h->ptab, h->ptab_g);
+                     tabE, tabW, h->ptab_g, h->pwcnt);
   fill(h, h->pstats, 0u, 4);
   HC(h, hipMemcpyAsync(h->pstats + 4, &h->prng, 4, hipMemcpyHostToDevice, h->stream));
   size_t lds = (size_t)h->nh * sizeof(float);
   hipEvent_t e = prof_begin(h, SOUP_PROF_PROTON);
-  hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, h->ptab,
-                     h->ptab_g, h->pstats + 4, h->pstats, h->pdstats);
+  hipLaunchKernelGGL(k_proton_chain, dim3(1), dim3(CHAIN_THREADS), lds, h->stream, h->c, h->b, h->nh, tabE, tabW,
+                     h->ptab_g, h->pwcnt, h->pstats + 4, h->pstats, h->pdstats);
   prof_end(h, SOUP_PROF_PROTON, e);
   uint32_t st[5];
   double ds;
@@ -736,6 +739,7 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
           DA(h, b.es, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab, (size_t)h->nh * cfg->es_cap);
           DA(h, h->ptab_g, (size_t)h->nh * cfg->es_cap);
+          DA(h, h->pwcnt, (size_t)h->nh);
         } else {
           DA(h, b.es, 1);
         }
