@@ -902,6 +902,160 @@ __device__ __forceinline__ void es_build_body(SoupConfig c, DevBuf b, uint32_t n
 }
 __global__ __launch_bounds__(256) void k_es_build(SoupConfig c, DevBuf b, uint32_t nh) { es_build_body(c, b, nh, blockIdx.x); }
 
+// ---- the same rows over a head cell grid --------------------------------------------------------
+// Cells of edge L / d >= (listRange + 0.02) / 2 per axis, d >= 5: every head within listRange + 0.01
+// of a head (the widest the test above admits) lies in the 5^3 cells around its own cell, each of
+// which the walk visits once. The heads are counting-sorted by cell (their order inside a cell is
+// free: the hits are gathered in a bitmap over head indices and written out in ascending order,
+// which is the row order of the all-pairs build).
+struct EsGrid {
+  uint32_t d[3];
+  float inv_w[3];  // d / L per axis
+  uint32_t* count;  // per cell
+  uint32_t* start;  // exclusive prefix, ncells + 1
+  float4* rec;      // heads by cell: position, w = head index bits
+};
+__device__ __forceinline__ uint32_t es_cell_axis(float x, float inv_w, uint32_t d) {
+  const int k = (int)floorf(x * inv_w);
+  return k < 0 ? 0u : (k >= (int)d ? d - 1u : (uint32_t)k);
+}
+__device__ __forceinline__ uint32_t es_cell_of(const EsGrid& g, float4 x) {
+  return es_cell_axis(x.x, g.inv_w[0], g.d[0]) + g.d[0] * (es_cell_axis(x.y, g.inv_w[1], g.d[1]) + g.d[1] * es_cell_axis(x.z, g.inv_w[2], g.d[2]));
+}
+__global__ __launch_bounds__(256) void k_es_cell_count(DevBuf b, uint32_t nh, EsGrid g) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nh) atomicAdd(&g.count[es_cell_of(g, b.head_q[h])], 1u);
+}
+// count must be zero again before this (it serves as the per-cell cursor)
+__global__ __launch_bounds__(256) void k_es_cell_fill(DevBuf b, uint32_t nh, EsGrid g) {
+  const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const float4 x = b.head_q[h];
+  const uint32_t cell = es_cell_of(g, x);
+  const uint32_t k = g.start[cell] + atomicAdd(&g.count[cell], 1u);
+  g.rec[k] = make_float4(x.x, x.y, x.z, __uint_as_float(h));
+}
+
+#define ES_CELL_WAVES 4u
+#ifndef ES_CELL_BATCH
+#define ES_CELL_BATCH 4u  // candidate chunks of 64 whose loads are issued together
+#endif
+// one wavefront per head, heads taken in cell order (neighbouring wavefronts share their stencils);
+// dynamic LDS: ES_CELL_WAVES bitmaps of ceil(nh / 32) words
+__global__ __launch_bounds__(ES_CELL_WAVES * WAVE) void k_es_cells(SoupConfig c, DevBuf b, uint32_t nh, EsGrid g) {
+  extern __shared__ uint32_t es_bits[];
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  const uint32_t k = uni(blockIdx.x * ES_CELL_WAVES + w);
+  if (k >= nh) return;  // no workgroup barrier below
+  const uint32_t nwords = (nh + 31u) / 32u;
+  uint32_t* bits = es_bits + (size_t)w * nwords;
+  for (uint32_t i = lane; i < nwords; i += WAVE) bits[i] = 0u;
+  const float4 xv = g.rec[k];
+  const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
+  const uint32_t hh = uni(__float_as_uint(xv.w));
+  const float lr = c.es_list_range, lr_hi = (lr + 0.01f) * (lr + 0.01f), lr_lo = (lr - 0.01f) * (lr - 0.01f);
+  const float ib0 = 1.0f / c.box[0], ib1 = 1.0f / c.box[1], ib2 = 1.0f / c.box[2];
+  // ---- the 125 stencil cells in canonical order, rows r = lane (A) and r = 64 + lane (B): first
+  // record and count; inclusive prefix of the counts over r ----
+  const int cx = (int)es_cell_axis(xi.x, g.inv_w[0], g.d[0]), cy = (int)es_cell_axis(xi.y, g.inv_w[1], g.d[1]),
+            cz = (int)es_cell_axis(xi.z, g.inv_w[2], g.d[2]);
+  uint32_t stA = 0, nA = 0, stB = 0, nB = 0;
+#pragma unroll
+  for (uint32_t half = 0; half < 2u; ++half) {
+    const uint32_t r = lane + half * WAVE;
+    if (r < 125u) {
+      int x = cx + (int)(r % 5u) - 2, y = cy + (int)((r / 5u) % 5u) - 2, z = cz + (int)(r / 25u) - 2;
+      x += x < 0 ? (int)g.d[0] : (x >= (int)g.d[0] ? -(int)g.d[0] : 0);
+      y += y < 0 ? (int)g.d[1] : (y >= (int)g.d[1] ? -(int)g.d[1] : 0);
+      z += z < 0 ? (int)g.d[2] : (z >= (int)g.d[2] ? -(int)g.d[2] : 0);
+      const uint32_t cell = (uint32_t)x + g.d[0] * ((uint32_t)y + g.d[1] * (uint32_t)z);
+      const uint32_t s0 = g.start[cell], s1 = g.start[cell + 1u];
+      if (half == 0u) { stA = s0; nA = s1 - s0; } else { stB = s0; nB = s1 - s0; }
+    }
+  }
+  uint32_t eA = nA, eB = nB;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    const uint32_t ya = (uint32_t)__shfl_up((int)eA, o, WAVE), yb = (uint32_t)__shfl_up((int)eB, o, WAVE);
+    if (lane >= (uint32_t)o) {
+      eA += ya;
+      eB += yb;
+    }
+  }
+  const uint32_t totA = uni((uint32_t)__builtin_amdgcn_readlane((int)eA, WAVE - 1));
+  eB += totA;
+  const uint32_t total = uni((uint32_t)__builtin_amdgcn_readlane((int)eB, WAVE - 1));
+  // offset (record index - flat index) of row r: start[r] - (end[r] - count[r])
+  const uint32_t oA = stA - (eA - nA), oB = stB - (eB - nB);
+  auto row_end = [&](uint32_t r) -> uint32_t {
+    return r < WAVE ? uni((uint32_t)__builtin_amdgcn_readlane((int)eA, (int)r)) : uni((uint32_t)__builtin_amdgcn_readlane((int)eB, (int)(r - WAVE)));
+  };
+  auto row_off = [&](uint32_t r) -> uint32_t {
+    return r < WAVE ? uni((uint32_t)__builtin_amdgcn_readlane((int)oA, (int)r)) : uni((uint32_t)__builtin_amdgcn_readlane((int)oB, (int)(r - WAVE)));
+  };
+  wave_lds_sync();  // the cleared bitmap before any hit lands in it
+  // ---- sweep the flat candidate range; the row boundaries inside each chunk are walked with
+  // uniform reads of the lanes holding them ----
+  uint32_t r = 0, rend = row_end(0), off = row_off(0);
+  for (uint32_t base = 0; base < total; base += ES_CELL_BATCH * WAVE) {
+    float4 xj[ES_CELL_BATCH];
+#pragma unroll
+    for (uint32_t q = 0; q < ES_CELL_BATCH; ++q) {
+      const uint32_t cb = base + q * WAVE, f = cb + lane;
+      uint32_t o = off;
+      while (rend < cb + WAVE && rend < total) {  // rows ending inside this chunk (empty ones at once)
+        const uint32_t pb = rend - cb;
+        ++r;
+        off = row_off(r);
+        if (lane >= pb) o = off;
+        rend = row_end(r);
+      }
+      xj[q] = g.rec[f < total ? f + o : k];
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < ES_CELL_BATCH; ++q) {
+      const uint32_t f = base + q * WAVE + lane;
+      const uint32_t gj = __float_as_uint(xj[q].w);
+      if (f < total && gj != hh) {
+        float ax = xi.x - xj[q].x, ay = xi.y - xj[q].y, az = xi.z - xj[q].z;
+        ax -= c.box[0] * rintf(ax * ib0);
+        ay -= c.box[1] * rintf(ay * ib1);
+        az -= c.box[2] * rintf(az * ib2);
+        const float a2 = ax * ax + ay * ay + az * az;
+        bool hit = a2 <= lr_lo;
+        if (!hit && a2 <= lr_hi) hit = dist_f(c, xi, make_float4(xj[q].x, xj[q].y, xj[q].z, 0.0f)) < lr;
+        if (hit) atomicOr(&bits[gj >> 5], 1u << (gj & 31u));
+      }
+    }
+  }
+  wave_lds_sync();
+  // ---- the row in ascending head order ----
+  const uint32_t cap = c.es_cap;
+  uint32_t* row = b.es + (size_t)hh * cap;
+  uint32_t count = 0;
+  for (uint32_t i0 = 0; i0 < nwords; i0 += WAVE) {
+    const uint32_t i = i0 + lane;
+    uint32_t word = i < nwords ? bits[i] : 0u;
+    const uint32_t pc = (uint32_t)__popc(word);
+    uint32_t incl = pc;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, o, WAVE);
+      if (lane >= (uint32_t)o) incl += y;
+    }
+    uint32_t p = count + incl - pc;
+    while (word) {
+      const uint32_t bit = (uint32_t)__builtin_ctz(word);
+      word &= word - 1u;
+      if (p < cap) row[p] = i * 32u + bit;
+      ++p;
+    }
+    count += uni((uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1));
+  }
+  if (lane == 0) {
+    b.es_count[hh] = count < cap ? count : cap;
+    if (count > cap) b.flags[1] = 1u;
+  }
+}
+
 // ================================================================================================
 // Forces: soup_force_list_main (verlet.wgsl:132-149) = bondedForce (step.wgsl:152-193) + the list
 // sum of nonbondedSoup (pair.wgsl:168-204), then soup_es_force_far_main (electrostatics-long.wgsl

@@ -62,6 +62,10 @@ struct SoupHandle {
   uint32_t prng = 0;
   uint32_t sweeps = 0;
   size_t ncells_cap = 0;     // cell_count / cell_start capacity (a box change may need more)
+  // head cell grid of the DH-list build (used when every axis has at least 5 cells)
+  EsGrid esg{};
+  bool es_grid = false;
+  size_t es_cells_cap = 0;
   // soup_potential_at scratch: positions in particle order, the configuration's cell grid
   float4* pa_xs = nullptr;
   uint32_t *pa_cell = nullptr, *pa_order = nullptr, *pa_count = nullptr, *pa_start = nullptr;
@@ -237,6 +241,39 @@ static void ph_drift(SoupHandle* h) {
 
 // encodeVerletRebuild (soup/src/soup-grid-verlet.ts:61-73): grid, DH list, Verlet list, snapshot,
 // drift reset -- here with the permutation of the state into the new cell order.
+// The DH-list build's head grid for the current box: cells of at least (listRange + 0.02) / 2 per
+// axis, so that the 5^3 cells around a head hold every head the list test can admit. Small boxes and
+// few heads keep the all-pairs build. Called outside every captured period (creation, box change).
+static int es_grid_setup(SoupHandle* h) {
+  h->es_grid = false;
+  if (!es_on(h) || h->nh < 256u) return 0;
+  const double w_min = 0.5 * ((double)h->c.es_list_range + 0.02);
+  size_t ncells = 1;
+  for (int a = 0; a < 3; ++a) {
+    const double L = (double)h->c.box[a];
+    const uint32_t d = (uint32_t)std::floor(L / w_min);
+    if (d < 5u) return 0;
+    h->esg.d[a] = d;
+    h->esg.inv_w[a] = (float)((double)d / L);
+    ncells *= d;
+  }
+  if (ncells > (size_t)64 * 1024 * 1024) return 0;
+  if (ncells > h->es_cells_cap) {
+    size_t cc = h->es_cells_cap, cs = h->es_cells_cap;
+    int r = dgrow(h, &h->esg.count, ncells, cc);
+    if (r) return r;
+    r = dgrow(h, &h->esg.start, ncells + 1, cs);
+    if (r) return r;
+    h->es_cells_cap = ncells;
+  }
+  if (!h->esg.rec) {
+    int r = dalloc(h, &h->esg.rec, h->nh);
+    if (r) return r;
+  }
+  h->es_grid = true;
+  return 0;
+}
+
 static void ph_rebuild(SoupHandle* h) {
   const uint32_t n = h->n;
   hipEvent_t e0 = prof_begin(h, SOUP_PROF_SORT);
@@ -265,7 +302,19 @@ static void ph_rebuild(SoupHandle* h) {
     hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
     scan(h, h->b.head_flag, n, h->b.head_pos);
     hipLaunchKernelGGL(k_head_rows, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
-    hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    if (h->es_grid) {
+      const uint32_t nc = h->esg.d[0] * h->esg.d[1] * h->esg.d[2];
+      fill(h, h->esg.count, 0u, nc);
+      hipLaunchKernelGGL(k_es_cell_count, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh, h->esg);
+      scan(h, h->esg.count, nc, h->esg.start);
+      fill(h, h->esg.count, 0u, nc);
+      hipLaunchKernelGGL(k_es_cell_fill, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh, h->esg);
+      const size_t lds = (size_t)ES_CELL_WAVES * ((h->nh + 31u) / 32u) * 4u;
+      hipLaunchKernelGGL(k_es_cells, dim3(blocks(h->nh, ES_CELL_WAVES)), dim3(ES_CELL_WAVES * WAVE), lds, h->stream, h->c, h->b,
+                         h->nh, h->esg);
+    } else {
+      hipLaunchKernelGGL(k_es_build, dim3(blocks((uint64_t)h->nh * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b, h->nh);
+    }
     prof_end(h, SOUP_PROF_ES_BUILD, e1);
   }
   hipEvent_t e2 = prof_begin(h, SOUP_PROF_VERLET_BUILD);
@@ -782,6 +831,10 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         h->global_step = init->global_step;
         h->next_sweep_at = init->next_sweep_at;
         h->prng = init->protonation_rng;
+        {
+          const int rg = es_grid_setup(h);
+          if (rg) return rg;
+        }
         // priming pass (soup/src/sim.ts:396-408)
         ph_rebuild(h);
         ph_force(h);
@@ -2260,6 +2313,8 @@ int soup_set_box(SoupHandle* h, const SoupConfig* geometry, const float* positio
   drop_graphs(h);  // their kernels carry the old geometry
   h->c = c;
   h->ncells = (uint32_t)ncells;
+  r = es_grid_setup(h);
+  if (r) return r;
   if (positions) {
     r = soup_write(h, SOUP_FIELD_POS, positions, (size_t)n * 16);
     if (r) return r;

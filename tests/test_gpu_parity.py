@@ -256,6 +256,36 @@ def test_charged_lists_far_force_and_sweep():
         s.dispose()
 
 
+def test_dh_lists_over_the_head_grid_match_oracle():
+    """A box of 42 sigma gives the DH-list build a head grid of 5 cells per axis (edge 8.4 >= (16.7 +
+    0.02) / 2), so the rows come from the 125-cell walk and the ordered bitmap instead of the all-pairs
+    scan: counts and rows (ascending head order) bit-exact against the oracle's all-pairs build, after
+    a rebuild at the creation state and after 60 steps of motion, and one constant-pH sweep on them."""
+    opts = charged(box=42.0, seed=3, pH=6.0, start={"C": 2000, "O": 2400, "H": 600, "M": 60, "W": 5000})
+    R = ps.resolve(opts)
+    box = float(R.cfg.box[0])
+    assert int(np.floor(box / (0.5 * (float(R.cfg.es_list_range) + 0.02)))) == 5
+    s = ps.createSoup(opts)
+    try:
+        for steps in (0, 60):
+            s.step(steps)
+            s.forces()
+            o = oracle_from_system(s, R)
+            o.rebuild()
+            nh = o.s.nh
+            assert nh == 2400
+            counts = s.readDEBUG("ES_COUNT")
+            assert np.array_equal(counts, o.es_count[:nh])
+            assert counts.mean() > 100
+            assert np.array_equal(s.readDEBUG("ES_LIST").reshape(nh, -1), o.es_list.reshape(-1, R.cfg.es_cap)[:nh])
+        r_gpu = s.protonationSweepDEBUG()
+        r_ref = o.protonation_sweep()
+        assert np.array_equal(s.charges(), o.charge) and s.protonationRngState() == o.s.prng
+        assert r_gpu["accepted"] == r_ref["accepted"]
+    finally:
+        s.dispose()
+
+
 def test_amphiphile_soup_runs():
     opts = amphiphile_opts(16.0, 200, 4, {"W": 2277}, seed=1)
     s = ps.createSoup(opts)
