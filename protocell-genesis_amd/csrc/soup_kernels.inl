@@ -313,12 +313,29 @@ __global__ __launch_bounds__(256) void k_relax_step(SoupConfig c, DevBuf b, floa
 // Rebuild: stable counting sort by (cell, particle index) + permutation of all state
 // (replaces clear/count/prefix/fill_main, engine/wgsl/neighbor.wgsl:26-64)
 // ================================================================================================
+// The slots are in cell order as of the last rebuild, so the lanes of a wavefront mostly share a
+// few cells: one atomic per distinct cell of the wavefront reserves that cell's ranks for all its
+// lanes (the order inside a cell is fixed afterwards by particle index, so any distinct ranks do).
 __device__ __forceinline__ void cell_count_body(SoupConfig c, DevBuf b, const uint32_t blk) {
-  uint32_t t = blk * blockDim.x + threadIdx.x;
-  if (t >= c.n) return;
-  uint32_t cell = cell_of(c, b.pos[t]);
-  b.cell_tmp[t] = cell;
-  b.cell_rank[t] = atomicAdd(&b.cell_count[cell], 1u);
+  const uint32_t t = blk * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & (WAVE - 1);
+  const bool live = t < c.n;
+  const uint32_t cell = live ? cell_of(c, b.pos[t]) : 0u;
+  if (live) b.cell_tmp[t] = cell;
+  uint64_t todo = __builtin_amdgcn_ballot_w64(live);
+  const uint64_t lower = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+  while (todo) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(todo);
+    const uint32_t lc = uni((uint32_t)__builtin_amdgcn_readlane((int)cell, (int)leader));
+    const uint64_t same = __builtin_amdgcn_ballot_w64(live && cell == lc);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&b.cell_count[lc], (uint32_t)__popcll(same));
+    base = uni((uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader));
+    if (live && cell == lc) rank = base + (uint32_t)__popcll(same & lower);
+    todo &= ~same;
+  }
+  if (live) b.cell_rank[t] = rank;
 }
 __global__ __launch_bounds__(256) void k_cell_count(SoupConfig c, DevBuf b) { cell_count_body(c, b, blockIdx.x); }
 
@@ -1333,6 +1350,8 @@ __device__ __forceinline__ void far_dh_body(SoupConfig c, DevBuf b, uint32_t nh,
     const PairConst P = make_pair_const(c);
     const uint32_t ecnt = uni(b.es_count[hh]);
     const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+    // every chunk of a batch is loaded and summed with no branch (a branch per chunk would make
+    // each load wait for the one before); entries past the count add nothing
     for (uint32_t base = 0; base < ecnt; base += FAR_BATCH * WAVE) {
       uint32_t v[FAR_BATCH];
 #pragma unroll
