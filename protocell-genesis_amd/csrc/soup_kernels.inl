@@ -748,21 +748,37 @@ __device__ __forceinline__ void verlet_row_staged(const SoupConfig& c, const Dev
     for (uint32_t k = 0; k < VL_BATCH; ++k) {
       const uint32_t kk = __float_as_uint(xj[k].w);
       // cheap test on the shifted position, the reference's exact one on the stored position in the
-      // narrow band between sure and cull
+      // narrow band between sure and cull (the squared distance is fused: the band is far wider than
+      // its rounding)
       const float ax = xj[k].x - xi.x, ay = xj[k].y - xi.y, az = xj[k].z - xi.z;
-      const float a2 = ax * ax + ay * ay + az * az;
-      bool hit = a2 <= sure;
-      if (__builtin_expect(!hit && a2 <= cull, 0)) hit = dist_f(c, xi, b.pos[kk]) < lr;
-      hit = hit && kk != t;
+      const float a2 = __builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az));
+      // the lane masks come straight from the compares (scalar registers): the sure hits, then the
+      // rare band resolved exactly
+      const bool sure_hit = a2 <= sure, band = !sure_hit && a2 <= cull;
+      const uint64_t m_sure = __builtin_amdgcn_fcmpf(a2, sure, 5 /* ordered <= */);
+      const uint64_t m_band = __builtin_amdgcn_fcmpf(a2, cull, 5 /* ordered <= */) & ~m_sure;
+      bool exact = false;
+      uint64_t m_exact = 0;
+      if (__builtin_expect(m_band != 0, 0)) {
+        if (band) exact = dist_f(c, xi, b.pos[kk]) < lr;
+        m_exact = __builtin_amdgcn_ballot_w64(exact);
+      }
+      const bool hit = (sure_hit || exact) && kk != t;
       const bool near = a2 < split2;
-      // ballots compact the hits in lane order: near ones from the front, far ones from the back
-      const uint64_t mh = __builtin_amdgcn_ballot_w64(hit), mn = __builtin_amdgcn_ballot_w64(hit && near);
-      const uint64_t mf = mh & ~mn;
+      const uint64_t m_hit = (m_sure | m_exact) & __builtin_amdgcn_uicmp(kk, t, 33 /* != */);
+      const uint64_t m_near = __builtin_amdgcn_fcmpf(a2, split2, 4 /* ordered < */);
+      // the hits compacted in lane order: near ones from the front, far ones from the back
+      const uint64_t mn = m_hit & m_near, mf = m_hit & ~m_near;
       const uint32_t pn = __builtin_amdgcn_mbcnt_hi((uint32_t)(mn >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mn, cn));
       const uint32_t pf = __builtin_amdgcn_mbcnt_hi((uint32_t)(mf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mf, cf));
-      // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row
-      const uint32_t pp = near ? pn : pf;
-      if (hit && pp < cap) row[near ? pp : cap - 1u - pp] = kk;
+      // past the capacity (an overflow, reported) the two parts overlap; every index stays in the row.
+      // While 64 more hits fit (uniform), no lane needs the capacity test.
+      const uint32_t idx = near ? pn : cap - 1u - pf;
+      if (cn + cf + WAVE <= cap) {
+        if (hit) row[idx] = kk;
+      } else if (hit && (near ? pn : pf) < cap) {
+        row[idx] = kk;
+      }
       cn += (uint32_t)__popcll(mn);
       cf += (uint32_t)__popcll(mf);
     }
