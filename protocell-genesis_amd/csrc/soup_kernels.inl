@@ -1139,6 +1139,13 @@ __device__ __forceinline__ PairConst make_pair_const(const SoupConfig& c) {
   return p;
 }
 
+// |d|^2 in one fixed order of operations (explicit fused multiply-adds, which the compiler does not
+// reorder). The list force and the far DH kernel split the DH term at es_split; both decide a pair's
+// side from this value of the same displacement, so every pair lands on exactly one side.
+__device__ __forceinline__ float r2_fixed(float dx, float dy, float dz) {
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
 // Minimum image with relative (not box-absolute) accuracy: for a pair straddling the periodic
 // boundary the box length is taken off the coordinate near L first, which is exact (Sterbenz), so
 // the small displacement carries only its own rounding. Positions lie in [0, L].
@@ -1181,7 +1188,7 @@ __device__ __forceinline__ void pair_force(const PairConst& P, const PairTab& T,
     dy = mi_acc(xi.y, xj.y, P.by, P.hy);
     dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
   }
-  float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-30f);
+  float r2 = fmaxf(r2_fixed(dx, dy, dz), 1e-30f);
   const float4 tb = T.e[((uint32_t)xi.w * SOUP_MAX_KINDS + (uint32_t)xj.w) & (SOUP_MAX_KINDS * SOUP_MAX_KINDS - 1u)];
   float ir2 = __builtin_amdgcn_rcpf(r2);
   float s2 = tb.y * ir2;
@@ -1260,7 +1267,7 @@ __device__ __forceinline__ float mi_sel(float a, float b, float L, float hL) {
 __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float dx, float dy, float dz, float qi, float qj,
                                           Acc& f) {
 #pragma clang fp contract(fast)
-  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float r2 = r2_fixed(dx, dy, dz);
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * ir;
   double fw = 0.0;
@@ -1293,20 +1300,36 @@ __device__ __forceinline__ void pair_body(const PairConst& P, float4 tb, float d
 
 // one far DH pair of soup_es_force_far_main (electrostatics-long.wgsl:63-82): only pairs at or beyond
 // the near/far split and inside the DH cutoff act; f32 terms from one reciprocal square root and one
-// hardware exp2 (relative error ~1e-7 over the cutoff range), FP64 sum
-__device__ __forceinline__ void far_pair_fast(const PairConst& P, float4 xi, float4 xj, float qi, float qj, Acc& f) {
+// hardware exp2 (relative error ~1e-7 over the cutoff range), summed per lane in f32 (a lane takes at
+// most a few dozen terms of a row) and over the lanes in FP64.
+// The minimum image is d - L rint(d / L), three instructions and no branch. It equals the list
+// force's displacement (mi_sel) for every pair that does not wrap, and is within a few 1e-6 sigma of
+// it for one that does; only the side of es_split depends on the last bit, so a wrapping pair within
+// 1e-4 of the split (relative, in r^2) takes mi_sel's exact displacement for that decision.
+struct AccF {
+  float x, y, z;
+};
+__device__ __forceinline__ void far_pair_lane(const PairConst& P, float ibx, float iby, float ibz, float4 xi, float4 xj,
+                                              float qi, float qj, AccF& f) {
 #pragma clang fp contract(fast)
+  const float ax = xi.x - xj.x, ay = xi.y - xj.y, az = xi.z - xj.z;
+  const float kx = __builtin_rintf(ax * ibx), ky = __builtin_rintf(ay * iby), kz = __builtin_rintf(az * ibz);
+  float dx = __builtin_fmaf(-P.bx, kx, ax), dy = __builtin_fmaf(-P.by, ky, ay), dz = __builtin_fmaf(-P.bz, kz, az);
+  float r2 = r2_fixed(dx, dy, dz);
+  if ((kx != 0.0f || ky != 0.0f || kz != 0.0f) && fabsf(r2 - P.split2) <= 1e-4f * P.split2) {
+    dx = mi_sel(xi.x, xj.x, P.bx, P.hx);
+    dy = mi_sel(xi.y, xj.y, P.by, P.hy);
+    dz = mi_sel(xi.z, xj.z, P.bz, P.hz);
+    r2 = r2_fixed(dx, dy, dz);
+  }
   const float qq = qi * qj;
-  const float dx = mi_sel(xi.x, xj.x, P.bx, P.hx), dy = mi_sel(xi.y, xj.y, P.by, P.hy), dz = mi_sel(xi.z, xj.z, P.bz, P.hz);
-  const float r2 = dx * dx + dy * dy + dz * dz;
   const float ir = __builtin_amdgcn_rsqf(r2);
   const float r = r2 * ir;
   const float u = P.es_A * __builtin_amdgcn_exp2f(P.es_kappa_l2 * r) * ir;
   const float fr = (qq != 0.0f && r2 >= P.split2 && r < P.es_rc) ? qq * (u * (ir + P.es_kappa) - P.es_Frc) * ir : 0.0f;
-  const double g = (double)fr;
-  f.x += g * (double)dx;
-  f.y += g * (double)dy;
-  f.z += g * (double)dz;
+  f.x = __builtin_fmaf(fr, dx, f.x);
+  f.y = __builtin_fmaf(fr, dy, f.y);
+  f.z = __builtin_fmaf(fr, dz, f.z);
 }
 
 // bondedForce split over four lanes: lane k < 3 takes bond slot k (FENE + the bends through that
@@ -1358,38 +1381,47 @@ __device__ __forceinline__ void far_dh_body(SoupConfig c, DevBuf b, uint32_t nh,
   const uint32_t r = uni(blk * (blockDim.x / WAVE) + w);
   if (r >= nh) return;
   const uint32_t hh = uni(b.head_rows[r]);
+  const uint32_t cap = c.es_cap;
+  const uint32_t* erow = b.es + (size_t)hh * cap;
+  // the first batch of row entries is requested with the head's own record and count, not after
+  // them: entries at or past the count (still inside the row's capacity) are read but never used
+  uint32_t v[FAR_BATCH];
+#pragma unroll
+  for (uint32_t k = 0; k < FAR_BATCH; ++k) {
+    const uint32_t s = k * WAVE + lane;
+    v[k] = s < cap ? erow[s] : hh;
+  }
   const float4 xv = b.head_q[hh];
+  const uint32_t ecnt = uni(b.es_count[hh]);
   const float4 xi = make_float4(uni_f(xv.x), uni_f(xv.y), uni_f(xv.z), 0.0f);
   const float qi = uni_f(xv.w);
-  Acc f{0.0, 0.0, 0.0};
+  AccF f{0.0f, 0.0f, 0.0f};
   if (qi != 0.0f) {
     const PairConst P = make_pair_const(c);
-    const uint32_t ecnt = uni(b.es_count[hh]);
-    const uint32_t* erow = b.es + (size_t)hh * c.es_cap;
+    const float ibx = 1.0f / P.bx, iby = 1.0f / P.by, ibz = 1.0f / P.bz;
     // every chunk of a batch is loaded and summed with no branch (a branch per chunk would make
     // each load wait for the one before); entries past the count add nothing
     for (uint32_t base = 0; base < ecnt; base += FAR_BATCH * WAVE) {
-      uint32_t v[FAR_BATCH];
+      if (base > 0) {
 #pragma unroll
-      for (uint32_t k = 0; k < FAR_BATCH; ++k) {
-        const uint32_t s = base + k * WAVE + lane;
-        v[k] = s < ecnt ? erow[s] : hh;
+        for (uint32_t k = 0; k < FAR_BATCH; ++k) {
+          const uint32_t s = base + k * WAVE + lane;
+          v[k] = s < ecnt ? erow[s] : hh;
+        }
       }
       float4 xj[FAR_BATCH];
 #pragma unroll
-      for (uint32_t k = 0; k < FAR_BATCH; ++k) xj[k] = b.head_q[v[k]];
+      for (uint32_t k = 0; k < FAR_BATCH; ++k) xj[k] = b.head_q[base + k * WAVE + lane < ecnt ? v[k] : hh];
 #pragma unroll
       for (uint32_t k = 0; k < FAR_BATCH; ++k)
-        far_pair_fast(P, xi, xj[k], qi, base + k * WAVE + lane < ecnt ? xj[k].w : 0.0f, f);
+        far_pair_lane(P, ibx, iby, ibz, xi, xj[k], qi, base + k * WAVE + lane < ecnt ? xj[k].w : 0.0f, f);
     }
   }
-  f.x = wave_sum_d_dpp(f.x);
-  f.y = wave_sum_d_dpp(f.y);
-  f.z = wave_sum_d_dpp(f.z);
+  const double sx = wave_sum_d_dpp((double)f.x), sy = wave_sum_d_dpp((double)f.y), sz = wave_sum_d_dpp((double)f.z);
   if (lane == 0) {
-    b.far_acc[3u * hh] = f.x;
-    b.far_acc[3u * hh + 1u] = f.y;
-    b.far_acc[3u * hh + 2u] = f.z;
+    b.far_acc[3u * hh] = sx;
+    b.far_acc[3u * hh + 1u] = sy;
+    b.far_acc[3u * hh + 2u] = sz;
   }
 }
 __global__ __launch_bounds__(256) void k_far_dh(SoupConfig c, DevBuf b, uint32_t nh) { far_dh_body(c, b, nh, xcd_block()); }
@@ -1498,7 +1530,7 @@ __device__ __forceinline__ void far_pair(const PairConst& P, float4 xi, float4 x
   float qq = qi * qj;
   if (qq == 0.0f) return;
   float dx = mi_acc(xi.x, xj.x, P.bx, P.hx), dy = mi_acc(xi.y, xj.y, P.by, P.hy), dz = mi_acc(xi.z, xj.z, P.bz, P.hz);
-  float r2 = dx * dx + dy * dy + dz * dz;
+  float r2 = r2_fixed(dx, dy, dz);
   if (r2 < P.split2) return;
   float r = sqrtf(r2);
   if (r >= P.es_rc) return;
