@@ -99,6 +99,8 @@ def drift_schedule(ps, torch, cadence_sys, steps):
     """The campaign under both Verlet rebuild schedules over the same number of steps (graphs on, the
     constant-pH sweep that falls inside included): the reference's fixed cadence on the benchmark's
     own system, and the displacement-triggered schedule on a second copy of the campaign."""
+    import numpy as np
+
     r0 = cadence_sys.rebuildCount()
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -118,7 +120,7 @@ def drift_schedule(ps, torch, cadence_sys, steps):
         torch.cuda.synchronize()
         t_drift = time.perf_counter() - t
         rd = d.rebuildCount() - r1
-        reads = d.readDEBUG("LIST_READS")
+        reads = d.readDEBUG("LIST_READS", np.uint64).astype(np.int64)
         cfg = d.verletConfig()
     finally:
         d.dispose()
@@ -127,7 +129,7 @@ def drift_schedule(ps, torch, cadence_sys, steps):
     return {"steps": steps, "cadence_steps_per_s": round(steps / t_cad, 1), "cadence_rebuilds": rc,
             "drift_steps_per_s": round(steps / t_drift, 1), "drift_rebuilds": rd,
             "drift_check_every": cfg["driftCheckEvery"], "drift_trigger": round(cfg["driftTrigger"], 4),
-            "drift_whole_row_force_fraction": round(float(reads[1]) / max(1, int(reads.sum())), 3)}
+            "drift_whole_row_force_fraction": round(float(reads[1]) / max(1, int(reads[0] + reads[1])), 3)}
 
 
 def cpu_baseline(sys_, R, sample_steps):
@@ -310,26 +312,31 @@ def main():
     # the list force's average launch, with HIP events on the library's stream, and the phase
     # breakdown: a separate, fully bracketed run of --phase-steps steps (every phase bracketed turns
     # graph replay off)
-    reads0 = s.readDEBUG("LIST_READS").astype(np.int64)
+    reads0 = s.readDEBUG("LIST_READS", np.uint64).astype(np.int64)
     s.profile(True)
     s.step(args.phase_steps)
     phases = s.profileRead()
     s.profile(False)
-    reads = s.readDEBUG("LIST_READS").astype(np.int64) - reads0  # near parts only, whole rows
+    # rows read as near parts only and whole; launches in which every row qualified for its near part,
+    # and the others (their particles also read their position at the rebuild to decide)
+    reads = s.readDEBUG("LIST_READS", np.uint64).astype(np.int64) - reads0
     occ = s.listOccupancyDEBUG()
 
     f = phases["force"]
     force_ms = f["ms"] / max(1, f["count"])
     lbar = occ["verletMean"]
     lnear = float(s.readDEBUG("VERLET_NEAR").mean())
-    frac_whole = float(reads[1]) / max(1, int(reads.sum()))
+    frac_whole = float(reads[1]) / max(1, int(reads[0] + reads[1]))  # of the rows a launch reads, those read whole
+    frac_test = float(reads[3]) / max(1, int(reads[2] + reads[3]))  # launches that test each particle's drift
     lread = frac_whole * lbar + (1.0 - frac_whole) * lnear  # Verlet entries a force launch reads
     n_heads = R.es_heads
     # algorithmic bytes of one list-force launch: per particle its own position, charge, counts, bond
-    # slots, centre link and force (60 B) and 4 B per Verlet entry it reads (the near part only while
-    # the drift allows, else the whole row); per charged head its head index and FP64 far DH sum
-    # (28 B); every neighbour gather is counted once per particle (the table is read, not re-read)
-    bytes_per_launch = R.N * (60.0 + 4.0 * lread) + n_heads * 28.0
+    # slots, centre link and force (60 B), 4 B per Verlet entry it reads (the near part while its own
+    # drift and the measured maximum allow, else the whole row) and, in launches that test each
+    # particle's drift, its position at the rebuild (16 B); per charged head its head index and FP64
+    # far DH sum (28 B); every neighbour gather is counted once per particle (the table is read, not
+    # re-read)
+    bytes_per_launch = R.N * (60.0 + 4.0 * lread + 16.0 * frac_test) + n_heads * 28.0
     achieved = bytes_per_launch / (force_ms * 1e-3) / 1e9
     # per step: the force launch, the list build's writes every 10 steps and the bond pass's reads of
     # the near parts every 20 steps, plus the per-particle state every phase streams; per head the
@@ -378,7 +385,7 @@ def main():
             "workload": "campaign soup 76 sigma, 483,268 beads, pH 7.0, 0.01 M, long-range DH, bond MC/20, rebuild/10, "
                         "pH sweep/1000, aggregate decomposition + encapsulated-water flood/1000",
             "N": R.N, "box": 76, "mean_verlet": round(lbar, 2), "mean_verlet_near": round(lnear, 2),
-            "force_launches_whole_rows": round(frac_whole, 4), "mean_dh_list": round(occ["esMean"], 2),
+            "force_rows_read_whole": round(frac_whole, 4), "force_launches_testing_drift": round(frac_test, 4), "mean_dh_list": round(occ["esMean"], 2),
             "pair_interactions_per_s": round(pair_per_step * steps_per_s, 1),
             "algorithmic_bytes_per_step": round(bytes_step),
             "step_roofline_frac": round(bytes_step * steps_per_s / (HBM_PEAK_GBS * 1e9), 4),

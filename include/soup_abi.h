@@ -157,8 +157,10 @@ enum {
   SOUP_FIELD_VERLET_NEAR = 21, /* u32 per particle: entries of the stored list row closer than the near/far
                                   split at the rebuild (no counterpart in the reference, whose row is one
                                   walk-ordered run; SOUP_FIELD_VERLET_LIST returns that run) */
-  SOUP_FIELD_LIST_READS = 22,  /* u32 x2 since creation: list-force evaluations that read the near parts
-                                  only, and those that read whole rows (no counterpart in the reference) */
+  SOUP_FIELD_LIST_READS = 22,  /* u64 x4 since creation: rows the list force read as the near part only,
+                                  rows it read whole (summed over its evaluations), evaluations in which
+                                  every row qualified for its near part, and the others (no counterpart in
+                                  the reference) */
   SOUP_FIELD_REBUILDS = 23     /* u64: Verlet rebuilds made by steps since creation, read-only (no counterpart
                                   in the reference) */
 };

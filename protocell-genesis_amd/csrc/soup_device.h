@@ -73,8 +73,11 @@ struct DevBuf {
   uint32_t* es_count;
   // counters: [0] verlet overflow, [1] es overflow, [2] max drift bits, [3] nonfinite pos,
   // [4] nonfinite vel, [5] bond event count, [6] bond resolution did not converge, [7] spare,
-  // [8] list-force launches that read the near parts only, [9] those that read whole rows
+  // [8] list-force launches in which every row qualified for its near part, [9] the others
   uint32_t* flags;
+  // list force, per wavefront of its grid: rows it read as the near part only, rows it read whole
+  // (pairs of counts, accumulated over launches)
+  uint32_t* rowreads;
   uint32_t* events;   // 2 * rules
   uint32_t* desorb;   // 2
   // bond MC scratch

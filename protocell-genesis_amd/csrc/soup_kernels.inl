@@ -1455,8 +1455,29 @@ __device__ __forceinline__ void force_group_body(SoupConfig c, DevBuf b, int es,
   const float4 xi = b.pos[t];
   const float qi = b.charge[t];
   const uint32_t cn = b.vnear[t];
-  const bool all = !near_only(c, b.flags[2]);
-  if (g == 0 && sub == 0) atomicAdd(&b.flags[all ? 9 : 8], 1u);
+  // A pair acting now was closer than reach + d_i + d_j at the rebuild (d = displacement since), and
+  // d_j <= dmax, the measured maximum: while d_i + dmax stays within the split's skin the near part of
+  // i's row holds every acting pair. Every row qualifies while 2 dmax does (near_only); otherwise each
+  // particle tests its own displacement, so only the rows of the particles that moved far are read
+  // whole.
+  const uint32_t dbits = b.flags[2];
+  const bool every_near = near_only(c, dbits);
+  if (g == 0 && sub == 0) atomicAdd(&b.flags[every_near ? 8 : 9], 1u);
+  bool all = false;
+  if (!every_near && valid) {
+    const float dm = sqrtf(__uint_as_float(dbits));
+    const float di = sqrtf(__uint_as_float(drift_sq_bits(c, xi, b.pos_at_rebuild[t])));
+    all = !(di + dm <= list_split(c) - 0.002f - force_reach(c) * 1.00005f);
+  }
+  {
+    const uint64_t lead = __builtin_amdgcn_ballot_w64(valid && sub == 0u);
+    const uint64_t whole = __builtin_amdgcn_ballot_w64(valid && sub == 0u && all);
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+      uint32_t* rr = b.rowreads + 2u * (blk * (blockDim.x / WAVE) + threadIdx.x / WAVE);
+      rr[0] += (uint32_t)__popcll(lead & ~whole);
+      rr[1] += (uint32_t)__popcll(whole);
+    }
+  }
   const uint32_t cnt = valid ? (all ? b.vcount[t] : cn) : 0u;
   const uint32_t cmax = uni(wave_max_u(cnt));
   const uint4 own = b.bs[t];
@@ -3051,6 +3072,38 @@ __global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, cons
 __global__ void k_vcount_to_index(const uint32_t* vcount, const uint32_t* id, uint32_t* out, uint32_t n) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) out[id[t]] = vcount[t];
+}
+// the list force's row reads summed over its per-wavefront counters: out[0] rows read as near parts,
+// out[1] rows read whole; out[2] launches in which every row qualified for its near part, out[3] the
+// others (flags[8], flags[9])
+__global__ __launch_bounds__(1024) void k_sum_rowreads(const uint32_t* rr, uint32_t nslots, const uint32_t* flags,
+                                                       unsigned long long* out) {
+  __shared__ unsigned long long s_part[2][1024 / WAVE];
+  unsigned long long a = 0, w = 0;
+  for (uint32_t k = threadIdx.x; k < nslots; k += blockDim.x) {
+    a += rr[2u * k];
+    w += rr[2u * k + 1u];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += (unsigned long long)__shfl_xor((long long)a, o, WAVE);
+    w += (unsigned long long)__shfl_xor((long long)w, o, WAVE);
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    s_part[0][threadIdx.x / WAVE] = a;
+    s_part[1][threadIdx.x / WAVE] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sa = 0, sw = 0;
+    for (uint32_t k = 0; k < blockDim.x / WAVE; ++k) {
+      sa += s_part[0][k];
+      sw += s_part[1][k];
+    }
+    out[0] = sa;
+    out[1] = sw;
+    out[2] = flags[8];
+    out[3] = flags[9];
+  }
 }
 // list occupancy: [0] sum verlet, [1] max verlet, [2] sum es, [3] max es (u64 sums in pairs)
 __global__ __launch_bounds__(256) void k_list_stats(const uint32_t* cnt, uint32_t n, unsigned long long* sum, uint32_t* mx) {

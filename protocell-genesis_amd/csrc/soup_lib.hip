@@ -88,6 +88,7 @@ struct SoupHandle {
   bool rebuild_due = false;
   uint64_t rebuilds = 0;  // list rebuilds made by steps (the priming and forced ones not counted)
   uint64_t gen = 0;       // bumped whenever the configuration, the particle count or a buffer changes
+  uint32_t rowread_slots = 0;  // wavefronts of the list force's largest grid (b.rowreads holds two counts each)
   bool use_graphs = true;
 };
 
@@ -761,6 +762,8 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         DA(h, b.vcount, n);
         DA(h, b.vnear, n);
         DA(h, b.flags, 10);
+        h->rowread_slots = xcd_blocks((uint64_t)n * FORCE_TPP, 256) * (256u / WAVE);
+        DA(h, b.rowreads, 2 * (size_t)h->rowread_slots);
         DA(h, b.events, 2 * SOUP_MAX_RULES);
         DA(h, b.desorb, 2);
         DA(h, b.props, (size_t)n * SOUP_CAND_CAP);
@@ -812,9 +815,11 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         a.head_q = b.head_q;
         a.far_acc = b.far_acc;
         a.gq = b.gq;
+        a.rowreads = b.rowreads;
         int u = upload_state(h, h->n, init);
         if (u) return u;
         HC(h, hipMemset(b.flags, 0, 40));
+        HC(h, hipMemset(b.rowreads, 0, 8 * (size_t)h->rowread_slots));
         uint32_t ev[2 * SOUP_MAX_RULES] = {0};
         if (init->events)
           for (uint32_t k = 0; k < 2 * cfg->n_rules && k < 2 * SOUP_MAX_RULES; ++k) ev[k] = init->events[k];
@@ -1518,7 +1523,7 @@ size_t soup_field_bytes(const SoupHandle* h, int field) {
     case SOUP_FIELD_ES_LIST: return (size_t)h->nh * h->c.es_cap * 4;
     case SOUP_FIELD_CELL_START: return ((size_t)h->ncells + 1) * 4;
     case SOUP_FIELD_MAX_DRIFT_SQ: return 4;
-    case SOUP_FIELD_LIST_READS: return 8;
+    case SOUP_FIELD_LIST_READS: return 32;
     case SOUP_FIELD_REBUILDS: return 8;
     case SOUP_FIELD_HEAD_IDX: return (size_t)h->nh * 4;
     default: return 0;
@@ -1570,7 +1575,11 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
     case SOUP_FIELD_EVENTS: src = h->b.events; break;
     case SOUP_FIELD_DESORB: src = h->b.desorb; break;
     case SOUP_FIELD_MAX_DRIFT_SQ: src = h->b.flags + 2; break;
-    case SOUP_FIELD_LIST_READS: src = h->b.flags + 8; break;
+    case SOUP_FIELD_LIST_READS:
+      hipLaunchKernelGGL(k_sum_rowreads, dim3(1), dim3(1024), 0, h->stream, h->b.rowreads, h->rowread_slots, h->b.flags,
+                         reinterpret_cast<unsigned long long*>(h->stage));
+      src = h->stage;
+      break;
     default: return fail(h, SOUP_E_INVALID, fmt("soup_read: unknown field %d", field));
   }
   HC(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
