@@ -345,14 +345,14 @@ def main():
     bytes_step = R.N * (249.4 + 4.0 * lread + 0.4 * lbar + 0.2 * lnear) + n_heads * (116.0 + 4.4 * occ["esMean"])
     pair_per_step = occ["verletPairs"] + occ["esPairs"]
     log("phase ms/launch: " + ", ".join(f"{k} {v['ms'] / max(1, v['count']):.3f}x{v['count']}" for k, v in phases.items() if v["count"]))
-    # measured HBM bytes of a launch that reads the near parts only and of one that reads whole rows
-    # (rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a copy calibration), weighted by this run's mix
+    # measured HBM bytes of a launch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same command's force
+    # dispatches, scaled by a copy calibration)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_force_group_campaign483k_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tj = json.load(fh)
-        traffic = round((1.0 - frac_whole) * tj["hbm_bytes_per_launch_near_rows"] + frac_whole * tj["hbm_bytes_per_launch_whole_rows"])
+        traffic = int(tj["hbm_bytes_per_launch"])
 
     schedules = drift_schedule(ps, torch, s, args.schedule_steps) if args.schedule_steps > 0 else None
 
