@@ -2,8 +2,8 @@
 // Langevin step on gfx950.
 //
 // Layout in HBM: every per-particle array is indexed by SLOT, the particle's position in
-// cell-sorted order at the last Verlet rebuild (cell-major, then ascending particle index inside a
-// cell). The rebuild permutes all per-particle state into the new slot order, so every per-step
+// cell-sorted order at the last Verlet rebuild (cell-major, then ascending x inside a cell, ties by
+// particle index). The rebuild permutes all per-particle state into the new slot order, so every per-step
 // kernel streams its arrays with unit stride and neighbour gathers stay spatially local. id[slot] is
 // the particle index (the reference's array index), inv[id] its current slot. Bond slots and
 // centerLink keep holding particle indices, exactly as in the reference.
@@ -11,8 +11,10 @@
 // The Verlet list is row-major, vl[slot * stride + s], with the stride (vl_stride) the capacity rounded
 // up to 32 entries so every row starts on its own 128-byte line. Entries are SLOTS (positions in the
 // same cell-sorted arrays). A row holds its near part (closer than the near/far split at the rebuild)
-// from the front in the canonical order of soup/wgsl/verlet.wgsl:66-88 (z-major cell walk, ascending
-// particle index inside each cell) and its far part from the back of the capacity in reverse order.
+// from the front in the cell order of soup/wgsl/verlet.wgsl:66-88 (z-major cell walk; inside a cell
+// ascending x, where the reference takes ascending particle index) and its far part from the back of
+// the capacity in reverse order. The list readback and the bond proposals order entries by the
+// reference's walk key (cell, then particle index).
 //
 // The DH list is row-major, es[h * es_cap + s]: row h is the h-th charged-kind particle in ascending
 // particle-index order (electrostatics-long.wgsl:15-29); its entries are the neighbour HEAD indices,
@@ -55,7 +57,6 @@ struct DevBuf {
   uint32_t* cell_start;
   uint32_t* cell_rank;
   uint32_t* cell_tmp;
-  uint32_t* perm;     // new slot -> old slot
   // lists
   uint32_t* vl;       // by slot: row of cap entries, near part from the front, far part from the back
   uint32_t* vcount;

@@ -310,12 +310,12 @@ __global__ __launch_bounds__(256) void k_relax_step(SoupConfig c, DevBuf b, floa
 }
 
 // ================================================================================================
-// Rebuild: stable counting sort by (cell, particle index) + permutation of all state
+// Rebuild: counting sort by (cell, x, particle index) + permutation of all state
 // (replaces clear/count/prefix/fill_main, engine/wgsl/neighbor.wgsl:26-64)
 // ================================================================================================
 // The slots are in cell order as of the last rebuild, so the lanes of a wavefront mostly share a
 // few cells: one atomic per distinct cell of the wavefront reserves that cell's ranks for all its
-// lanes (the order inside a cell is fixed afterwards by particle index, so any distinct ranks do).
+// lanes (the order inside a cell is fixed afterwards by x and particle index, so any distinct ranks do).
 __device__ __forceinline__ void cell_count_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   const uint32_t t = blk * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -339,23 +339,41 @@ __device__ __forceinline__ void cell_count_body(SoupConfig c, DevBuf b, const ui
 }
 __global__ __launch_bounds__(256) void k_cell_count(SoupConfig c, DevBuf b) { cell_count_body(c, b, blockIdx.x); }
 
-// unordered fill into the cell segment, then rank by particle index inside the segment
-__device__ __forceinline__ void cell_fill_body(SoupConfig c, DevBuf b, const uint32_t blk) {
-  uint32_t t = blk * blockDim.x + threadIdx.x;
-  if (t >= c.n) return;
-  b.perm[b.cell_start[b.cell_tmp[t]] + b.cell_rank[t]] = t;  // perm used as scratch here
+// The order inside a cell: ascending x, ties by particle index (a fixed order; the Verlet build trims
+// the two end cells of each stencil row to the particle's x-chord with it). The reference orders a
+// cell by particle index; the list readback and the bond proposals restore that walk order by key.
+// x as an unsigned key in the order of the floats (a total order: a non-finite x, which the chunk-end
+// guard reports, still gets a rank of its own)
+__device__ __forceinline__ uint32_t x_order_key(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-__global__ __launch_bounds__(256) void k_cell_fill(SoupConfig c, DevBuf b) { cell_fill_body(c, b, blockIdx.x); }
-
-__device__ __forceinline__ void cell_rank_body(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old, const uint32_t blk) {
+// unordered fill of the sort keys (x key, particle index) into the cell segment (keys: n entries of
+// scratch), then each particle's rank among its segment's keys
+__device__ __forceinline__ void cell_fill_body(SoupConfig c, DevBuf b, uint2* keys, const uint32_t blk) {
   uint32_t t = blk * blockDim.x + threadIdx.x;
   if (t >= c.n) return;
-  uint32_t cell = b.cell_tmp[t], me = b.id[t];
-  uint32_t s0 = b.cell_start[cell], s1 = b.cell_start[cell + 1], r = 0;
-  for (uint32_t k = s0; k < s1; ++k) r += (b.id[b.perm[k]] < me) ? 1u : 0u;
+  keys[b.cell_start[b.cell_tmp[t]] + b.cell_rank[t]] = make_uint2(x_order_key(b.pos[t].x), b.id[t]);
+}
+__global__ __launch_bounds__(256) void k_cell_fill(SoupConfig c, DevBuf b, uint2* keys) { cell_fill_body(c, b, keys, blockIdx.x); }
+
+__device__ __forceinline__ void cell_rank_body(SoupConfig c, DevBuf b, const uint2* keys, uint32_t* new_slot_of_old,
+                                               const uint32_t blk) {
+  uint32_t t = blk * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t cell = b.cell_tmp[t];
+  const uint32_t s0 = b.cell_start[cell], s1 = b.cell_start[cell + 1];
+  const uint2 me = keys[s0 + b.cell_rank[t]];
+  uint32_t r = 0;
+  for (uint32_t k = s0; k < s1; ++k) {
+    const uint2 o = keys[k];
+    r += (o.x < me.x || (o.x == me.x && o.y < me.y)) ? 1u : 0u;
+  }
   new_slot_of_old[t] = s0 + r;
 }
-__global__ __launch_bounds__(256) void k_cell_rank(SoupConfig c, DevBuf b, uint32_t* new_slot_of_old) { cell_rank_body(c, b, new_slot_of_old, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_cell_rank(SoupConfig c, DevBuf b, const uint2* keys, uint32_t* new_slot_of_old) {
+  cell_rank_body(c, b, keys, new_slot_of_old, blockIdx.x);
+}
 
 // gathers every per-particle array into the new slot order (out = the alternate buffers)
 __device__ __forceinline__ void permute_body(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old, const uint32_t blk) {
@@ -438,13 +456,37 @@ __device__ __forceinline__ bool near_only(const SoupConfig& c, uint32_t drift_sq
 }
 // entry s of a row read as near part then far part (n = near entries)
 __device__ __forceinline__ uint32_t row_index(uint32_t s, uint32_t n, uint32_t cap) { return s < n ? s : cap + n - 1u - s; }
+// cell of list entry u of particle t in the walk order: the cell offset (dz, dy, dx), each wrapped
+// into -R..R, as one number
+__device__ __forceinline__ uint32_t walk_cell(const SoupConfig& c, const uint32_t* cell_of, uint32_t ct, uint32_t u) {
+  const int R = (int)c.walk_radius, side = 2 * R + 1;
+  uint32_t a = ct, q = cell_of[u];
+  int key = 0, scale = 1;
+  for (int k = 0; k < 3; ++k) {
+    const int D = (int)c.dims[k];
+    int v = (int)(q % (uint32_t)D) - (int)(a % (uint32_t)D);
+    if (v > R) v -= D;
+    if (v < -R) v += D;
+    key += (v + R) * scale;
+    scale *= side;
+    a /= (uint32_t)D;
+    q /= (uint32_t)D;
+  }
+  return (uint32_t)key;
+}
+// position of list entry u (particle index j) in the reference's walk order: its cell, then the
+// particle index (the reference orders each cell by index)
+__device__ __forceinline__ uint64_t walk_key(const SoupConfig& c, const uint32_t* cell_of, uint32_t ct, uint32_t u, uint32_t j) {
+  return ((uint64_t)walk_cell(c, cell_of, ct, u) << 32) | j;
+}
 
 // ================================================================================================
 // Verlet list: soup_build_verlet_list_main (soup/wgsl/verlet.wgsl:53-92), one wavefront per particle.
-// Canonical order: cells z-major (dz, dy, dx in -R..R), ascending particle index inside a cell. Lane r
+// Canonical order: cells z-major (dz, dy, dx in -R..R), ascending x inside a cell. Lane r
 // first sets up row r = (dz, dy): cells lying entirely beyond listRange + 0.01 sigma of the particle
 // are dropped (they hold no entry, so the list is unchanged), which leaves one interval of dx around
-// dx = 0; a scan over the lanes lays the kept ranges end to end, and the wavefront sweeps that one
+// dx = 0 (from the staging, the interval's two end cells are trimmed to the x-chord of the culling
+// sphere as well); a scan over the lanes lays the kept ranges end to end, and the wavefront sweeps that one
 // dense candidate range, VL_BATCH chunks of 64 at a time (each lane carries its row from chunk to
 // chunk, since its index only grows). Ballots compact the hits in order: the near ones (list_split)
 // from the front of the row, the far ones from its back. Needs (2R + 1)^2 <= 64 rows.
@@ -667,15 +709,35 @@ __device__ __forceinline__ VlShared verlet_shared(const SoupConfig& c, int cx, i
   const float split = list_split(c);
   v.split2 = split * split;
   const int R = (int)c.walk_radius, side = 2 * R + 1;
-  v.row_ok = lane < (uint32_t)(side * side);
-  const int dz = (int)lane / side - R, dy = (int)lane % side - R;
+  // up to 32 rows (R <= 2): lanes r and r + 32 both hold row r, and split the row's two searches
+  const uint32_t nrow = (uint32_t)(side * side);
+  const uint32_t rl = nrow <= WAVE / 2 ? (lane & (WAVE / 2 - 1)) : lane;
+  v.row_ok = rl < nrow;
+  const int dz = (int)rl / side - R, dy = (int)rl % side - R;
   // gap to the row's cells: dz > 0: dz w - lo, dz < 0: lo - (dz + 1) w... written as a + s lo
   v.az = dz > 0 ? (float)dz * w2 : (dz < 0 ? (float)(-dz - 1) * w2 : 0.0f);
   v.sz = dz > 0 ? -1.0f : (dz < 0 ? 1.0f : 0.0f);
   v.ay = dy > 0 ? (float)dy * w1 : (dy < 0 ? (float)(-dy - 1) * w1 : 0.0f);
   v.sy = dy > 0 ? -1.0f : (dy < 0 ? 1.0f : 0.0f);
-  v.k_row = lane * (uint32_t)side;
+  v.k_row = rl * (uint32_t)side;
   return v;
+}
+
+// first staged index in [a, e) whose x exceeds thr (past_equal) or is not below it; the staged x
+// ascend inside one cell
+__device__ __forceinline__ uint32_t staged_x_bound(const float4* s_pos, uint32_t a, uint32_t e, float thr, bool past_equal) {
+  uint32_t n = e - a;
+  while (n != 0u) {
+    const uint32_t half = n >> 1;
+    const float xv = s_pos[a + half].x;
+    if (xv < thr || (past_equal && xv == thr)) {
+      a += half + 1u;
+      n -= half + 1u;
+    } else {
+      n = half;
+    }
+  }
+  return a;
 }
 
 // particle t's row from the stencil staged in LDS (s_pos, with s_cpre the exclusive prefix of the
@@ -694,28 +756,48 @@ __device__ __forceinline__ void verlet_row_staged(const SoupConfig& c, const Dev
   const float hi0 = v.w0 - lo0;
   const float cull = v.cull, sure = v.sure, lr = v.lr;
   const int R = (int)c.walk_radius;
-  // ---- row set-up, lane r = row (dz, dy) in canonical order: its kept cells dx = dxl..dxh are the
-  // stencil cells r * side + R + dxl .. r * side + R + dxh, one contiguous range of the staging ----
+  // ---- row set-up, lane r = row (dz, dy) in canonical order (and lane r + 32 too when there are at
+  // most 32 rows): its kept cells dx = dxl..dxh are the stencil cells r * side + R + dxl ..
+  // r * side + R + dxh, one contiguous range of the staging ----
   uint32_t beg = 0, len = 0;
   const float gz = fmaxf(v.az + v.sz * lo2, 0.0f), gy = fmaxf(v.ay + v.sy * lo1, 0.0f);
   const float gyz = gy * gy + gz * gz;
-  if (v.row_ok && gyz <= cull) {
+  const bool pair_lanes = (uint32_t)((2 * R + 1) * (2 * R + 1)) <= WAVE / 2;
+  const bool live = v.row_ok && gyz <= cull;
+  uint32_t lo_at = 0, hi_at = 0;  // the trimmed range [lo_at, hi_at) of the row's kept cells
+  if (live) {
     int dxl = 0, dxh = 0;
     for (int dx = 1; dx <= R; ++dx) {
       const float gh = fmaxf((float)(dx - 1) * v.w0 + hi0, 0.0f), gl = fmaxf((float)(dx - 1) * v.w0 + lo0, 0.0f);
       if (gh * gh + gyz <= cull) dxh = dx;
       if (gl * gl + gyz <= cull) dxl = -dx;
     }
-    const uint32_t k0 = v.k_row + (uint32_t)(R + dxl);
-    beg = s_cpre[k0];
-    len = s_cpre[k0 + (uint32_t)(dxh - dxl) + 1u] - beg;
+    // the first and last kept cells, each sorted by x, trimmed to the row's x-chord |x_j - x_i| <= h
+    // of the culling sphere (h^2 = cull - gyz): binary searches in the staging. A trimmed bead lies
+    // beyond listRange + the margin less rounding, as a culled cell does
+    const uint32_t k0 = v.k_row + (uint32_t)(R + dxl), k1 = k0 + (uint32_t)(dxh - dxl);
+    const float h = sqrtf(cull - gyz), xlo = xi.x - h, xhi = xi.x + h;
+    if (pair_lanes) {  // lane r: the first cell's lower bound, lane r + 32: the last cell's upper bound
+      const bool up = lane >= WAVE / 2;
+      const uint32_t k = up ? k1 : k0;
+      lo_at = staged_x_bound(s_pos, s_cpre[k], s_cpre[k + 1u], up ? xhi : xlo, up);
+    } else {
+      lo_at = staged_x_bound(s_pos, s_cpre[k0], s_cpre[k0 + 1u], xlo, false);
+      hi_at = staged_x_bound(s_pos, s_cpre[k1], s_cpre[k1 + 1u], xhi, true);
+    }
   }
-  uint32_t end = len;  // inclusive prefix over the lanes: end[r]
-  for (int o = 1; o < WAVE; o <<= 1) {
+  if (pair_lanes) hi_at = (uint32_t)__shfl_down((int)lo_at, WAVE / 2, WAVE);
+  if (live && (!pair_lanes || lane < WAVE / 2)) {
+    beg = lo_at;
+    len = hi_at > lo_at ? hi_at - lo_at : 0u;
+  }
+  uint32_t end = len;  // inclusive prefix over the row lanes: end[r]
+  const int nlanes = pair_lanes ? WAVE / 2 : WAVE;
+  for (int o = 1; o < nlanes; o <<= 1) {
     const uint32_t y = (uint32_t)__shfl_up((int)end, o, WAVE);
     if (lane >= (uint32_t)o) end += y;
   }
-  const uint32_t total = uni((uint32_t)__builtin_amdgcn_readlane((int)end, WAVE - 1));
+  const uint32_t total = uni((uint32_t)__builtin_amdgcn_readlane((int)end, nlanes - 1));
   // ---- dense sweep ----
   const uint32_t cap = c.list_cap;
   uint32_t* row = b.vl + (size_t)t * vl_stride(c);
@@ -835,32 +917,56 @@ __device__ __forceinline__ void verlet_cell_body(SoupConfig c, DevBuf b, const u
     // far-away padding after the stencil: the sweep reads up to VL_PAD entries past it
     for (uint32_t e = threadIdx.x; e < VL_PAD; e += VL_WAVES * WAVE)
       s_pos[total + e] = make_float4(3.0e37f, 3.0e37f, 3.0e37f, 0.0f);
-    // ---- stage: wavefront w copies stencil cells w, w + VL_WAVES, ..., eight cells' loads at a time ----
-    for (uint32_t k0 = w; k0 < nst; k0 += 8u * VL_WAVES) {
-      float4 p[8];
-      uint32_t cs[8], code[8], n[8], dst[8];
-#pragma unroll
-      for (uint32_t j = 0; j < 8u; ++j) {
-        const uint32_t k = k0 + j * VL_WAVES;
-        n[j] = cs[j] = code[j] = dst[j] = 0u;
-        if (k < nst) {
-          const uint2 ci = s_cinfo[k];
-          cs[j] = uni(ci.x);
-          code[j] = uni(ci.y);
-          dst[j] = uni(s_cpre[k]);
-          n[j] = uni(s_cpre[k + 1u]) - dst[j];
-        }
-        if (lane < n[j]) p[j] = b.pos[cs[j] + lane];
+    // ---- stage: wavefront w copies stencil rows w, w + VL_WAVES, ... A row's side cells follow each
+    // other in x, so each run of cells with one x image shift is one contiguous slot range (one run,
+    // two where the row wraps in x, three on a grid narrower than the stencil); the lanes stream the
+    // row's beads four loads at a time ----
+    const uint32_t nrow = (uint32_t)(side * side);
+    for (uint32_t q = w; q < nrow; q += VL_WAVES) {
+      const uint32_t k = q * (uint32_t)side;
+      uint32_t slot0 = 0, code = 0, pre = 0;
+      if (lane <= (uint32_t)side) pre = s_cpre[k + lane];
+      if (lane < (uint32_t)side) {
+        const uint2 ci = s_cinfo[k + lane];
+        slot0 = ci.x;
+        code = ci.y;
       }
+      const uint32_t cxl = (uint32_t)__shfl_up((int)(code & 3u), 1, WAVE);
+      uint64_t splits = __ballot(lane > 0u && lane < (uint32_t)side && (code & 3u) != cxl);
+      const uint32_t j1 = splits ? (uint32_t)__builtin_ctzll(splits) : (uint32_t)side;
+      splits &= splits - 1ull;
+      const uint32_t j2 = splits ? (uint32_t)__builtin_ctzll(splits) : (uint32_t)side;
+      const uint32_t e0 = uni((uint32_t)__builtin_amdgcn_readlane((int)pre, 0));
+      const uint32_t n = uni((uint32_t)__builtin_amdgcn_readlane((int)pre, side)) - e0;
+      const uint32_t E1 = uni((uint32_t)__builtin_amdgcn_readlane((int)pre, (int)j1)) - e0;
+      const uint32_t E2 = uni((uint32_t)__builtin_amdgcn_readlane((int)pre, (int)j2)) - e0;
+      // slot = rel + B (rel: the bead's offset in the row), per run
+      const uint32_t B0 = uni((uint32_t)__builtin_amdgcn_readlane((int)slot0, 0));
+      const uint32_t B1 = uni((uint32_t)__builtin_amdgcn_readlane((int)slot0, (int)(j1 < (uint32_t)side ? j1 : 0u))) - E1;
+      const uint32_t B2 = uni((uint32_t)__builtin_amdgcn_readlane((int)slot0, (int)(j2 < (uint32_t)side ? j2 : 0u))) - E2;
+      const uint32_t c0 = uni((uint32_t)__builtin_amdgcn_readlane((int)code, 0));
+      const uint32_t c1 = uni((uint32_t)__builtin_amdgcn_readlane((int)code, (int)(j1 < (uint32_t)side ? j1 : 0u)));
+      const uint32_t c2 = uni((uint32_t)__builtin_amdgcn_readlane((int)code, (int)(j2 < (uint32_t)side ? j2 : 0u)));
+      const float X0 = (c0 & 1u) ? -c.box[0] : ((c0 & 2u) ? c.box[0] : 0.0f);
+      const float X1 = (c1 & 1u) ? -c.box[0] : ((c1 & 2u) ? c.box[0] : 0.0f);
+      const float X2 = (c2 & 1u) ? -c.box[0] : ((c2 & 2u) ? c.box[0] : 0.0f);
+      const float sy = (c0 & 4u) ? -c.box[1] : ((c0 & 8u) ? c.box[1] : 0.0f);
+      const float sz = (c0 & 16u) ? -c.box[2] : ((c0 & 32u) ? c.box[2] : 0.0f);
+      for (uint32_t r0 = 0; r0 < n; r0 += 4u * WAVE) {
+        float4 p[4];
+        uint32_t sl[4];
+        float sx[4];
 #pragma unroll
-      for (uint32_t j = 0; j < 8u; ++j) {
-        const float sx = (code[j] & 1u) ? -c.box[0] : ((code[j] & 2u) ? c.box[0] : 0.0f);
-        const float sy = (code[j] & 4u) ? -c.box[1] : ((code[j] & 8u) ? c.box[1] : 0.0f);
-        const float sz = (code[j] & 16u) ? -c.box[2] : ((code[j] & 32u) ? c.box[2] : 0.0f);
-        if (lane < n[j]) s_pos[dst[j] + lane] = make_float4(p[j].x + sx, p[j].y + sy, p[j].z + sz, __uint_as_float(cs[j] + lane));
-        for (uint32_t e = lane + WAVE; e < n[j]; e += WAVE) {  // cells of more than 64 beads
-          const float4 q = b.pos[cs[j] + e];
-          s_pos[dst[j] + e] = make_float4(q.x + sx, q.y + sy, q.z + sz, __uint_as_float(cs[j] + e));
+        for (uint32_t u = 0; u < 4u; ++u) {
+          const uint32_t rel = r0 + u * WAVE + lane;
+          sl[u] = rel + (rel >= E2 ? B2 : (rel >= E1 ? B1 : B0));
+          sx[u] = rel >= E2 ? X2 : (rel >= E1 ? X1 : X0);
+          if (rel < n) p[u] = b.pos[sl[u]];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u) {
+          const uint32_t rel = r0 + u * WAVE + lane;
+          if (rel < n) s_pos[e0 + rel] = make_float4(p[u].x + sx[u], p[u].y + sy, p[u].z + sz, __uint_as_float(sl[u]));
         }
       }
     }
@@ -1606,7 +1712,7 @@ __device__ __forceinline__ uint32_t role_of(const SoupConfig& c, uint32_t r, flo
 }
 
 // one wavefront per particle: the lanes test 64 list entries at a time, and ballots keep the first
-// catalyst within reach and the first SOUP_CAND_CAP candidates in list order, as the serial walk does
+// catalyst within reach and the first SOUP_CAND_CAP candidates in the reference's walk order
 __device__ __forceinline__ void bond_propose_body(SoupConfig c, DevBuf b, const uint32_t blk) {
   const uint32_t lane = threadIdx.x & (WAVE - 1);
   const uint32_t t = blk * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -1627,18 +1733,24 @@ __device__ __forceinline__ void bond_propose_body(SoupConfig c, DevBuf b, const 
   const uint32_t i = b.id[t];
   const uint4 own = b.bs[t];
   const float rcat = wca_cut(c, pair_b(c, ti, c.catalyst_kind));
-  uint32_t catNear = 0, catId = SOUP_NONE, nCand = 0;
-  uint32_t myJ = SOUP_NONE, myR = 0;  // candidate k lives in lane k
-  // every candidate and catalyst lies in the near part of the row, which keeps the walk order
-  const uint32_t cnt = b.vnear[t];
+  // Every candidate and catalyst lies in the near part of the row. The near part visits the cells in
+  // walk order but orders each cell by x, where the reference orders it by particle index, so the
+  // choice goes by walk key (cell, then index): the catalyst of least key, and a running list of the
+  // least-key candidates (lane k holds the k-th), each new candidate inserted at its rank. Once the
+  // running choice lies in cells before the last cell a chunk reached, later entries cannot change it.
+  uint32_t catId = SOUP_NONE, nCand = 0;
+  uint64_t catKey = ~0ull;
+  uint32_t myJ = SOUP_NONE, myR = 0;
+  uint64_t myKey = ~0ull;
+  const uint32_t cnt = b.vnear[t], ct = b.cell_of[t];
   const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
-  for (uint32_t base = 0; base < cnt && (catNear == 0u || nCand < SOUP_CAND_CAP); base += WAVE) {
+  for (uint32_t base = 0; base < cnt; base += WAVE) {
     const uint32_t s = base + lane;
     bool isCat = false, isCand = false;
-    uint32_t j = 0;
+    uint32_t j = 0, u = 0;
     int r = -1;
     if (s < cnt) {
-      const uint32_t u = row[s];
+      u = row[s];
       const float4 xj = b.pos[u];
       const float tj = xj.w;
       j = b.id[u];
@@ -1648,20 +1760,49 @@ __device__ __forceinline__ void bond_propose_body(SoupConfig c, DevBuf b, const 
         if (r >= 0 && own.x != j && own.y != j && own.z != j) isCand = dist_f(c, xi, xj) < wca_cut(c, pair_b(c, ti, tj));
       }
     }
-    const uint64_t cm = __ballot(isCat);
-    if (catNear == 0u && cm != 0ull) {
-      catId = (uint32_t)__shfl((int)j, __ffsll((unsigned long long)cm) - 1, WAVE);
-      catNear = 1u;
+    const uint32_t kc = (isCat || isCand) ? walk_cell(c, b.cell_of, ct, u) : 0u;
+    uint64_t cm = __ballot(isCat);
+    while (cm != 0ull) {  // catalysts in contact: a few at most
+      const int k = __ffsll((unsigned long long)cm) - 1;
+      const uint64_t kk = ((uint64_t)(uint32_t)__shfl((int)kc, k, WAVE) << 32) | (uint32_t)__shfl((int)j, k, WAVE);
+      if (kk < catKey) {
+        catKey = kk;
+        catId = (uint32_t)kk;
+      }
+      cm &= cm - 1ull;
     }
     uint64_t dm = __ballot(isCand);
-    while (dm != 0ull && nCand < SOUP_CAND_CAP) {
+    while (dm != 0ull) {
       const int k = __ffsll((unsigned long long)dm) - 1;
       const uint32_t cj = (uint32_t)__shfl((int)j, k, WAVE), cr = (uint32_t)__shfl(r, k, WAVE);
-      if (lane == nCand) { myJ = cj; myR = cr; }
-      ++nCand;
+      const uint64_t kk = ((uint64_t)(uint32_t)__shfl((int)kc, k, WAVE) << 32) | cj;
+      const uint32_t p = (uint32_t)__popcll(__ballot(lane < nCand && myKey < kk));
+      if (p < SOUP_CAND_CAP) {
+        const uint32_t upJ = (uint32_t)__shfl_up((int)myJ, 1, WAVE), upR = (uint32_t)__shfl_up((int)myR, 1, WAVE);
+        const uint64_t upK = ((uint64_t)(uint32_t)__shfl_up((int)(myKey >> 32), 1, WAVE) << 32) |
+                             (uint32_t)__shfl_up((int)(uint32_t)myKey, 1, WAVE);
+        if (lane > p) {
+          myJ = upJ;
+          myR = upR;
+          myKey = upK;
+        }
+        if (lane == p) {
+          myJ = cj;
+          myR = cr;
+          myKey = kk;
+        }
+        nCand = nCand < SOUP_CAND_CAP ? nCand + 1u : SOUP_CAND_CAP;
+      }
       dm &= dm - 1ull;
     }
+    if (catId != SOUP_NONE && nCand == SOUP_CAND_CAP) {
+      const uint32_t last = (base + WAVE < cnt ? base + WAVE : cnt) - 1u;
+      const uint32_t lastCell = uni(walk_cell(c, b.cell_of, ct, row[last]));
+      const uint32_t candCell = (uint32_t)__shfl((int)(myKey >> 32), (int)SOUP_CAND_CAP - 1, WAVE);
+      if ((uint32_t)(catKey >> 32) < lastCell && candCell < lastCell) break;
+    }
   }
+  const uint32_t catNear = catId != SOUP_NONE ? 1u : 0u;
   // bondFormDecide draws (bond-dispatch.wgsl:104-112), serial in candidate order on every lane
   uint32_t rng = b.brng[t], nacc = 0;
   for (uint32_t k = 0; k < nCand; ++k) {
@@ -3023,43 +3164,39 @@ __global__ void k_bs_from_index(const uint32_t* src, const uint32_t* id, uint4* 
   uint32_t i = id[t];
   bs[t] = make_uint4(src[3u * i], src[3u * i + 1], src[3u * i + 2], 0u);
 }
-// position of list entry u of particle t in the walk order: the cell offset (dz, dy, dx), each
-// wrapped into -R..R, then the slot (ascending particle index inside a cell)
-__device__ __forceinline__ uint64_t walk_key(const SoupConfig& c, const uint32_t* cell_of, uint32_t ct, uint32_t u) {
-  const int R = (int)c.walk_radius, side = 2 * R + 1;
-  uint32_t a = ct, q = cell_of[u];
-  int key = 0, scale = 1;
-  for (int k = 0; k < 3; ++k) {
-    const int D = (int)c.dims[k];
-    int v = (int)(q % (uint32_t)D) - (int)(a % (uint32_t)D);
-    if (v > R) v -= D;
-    if (v < -R) v += D;
-    key += (v + R) * scale;
-    scale *= side;
-    a /= (uint32_t)D;
-    q /= (uint32_t)D;
-  }
-  return ((uint64_t)key << 32) | u;
-}
-// Verlet list in the reference layout: out[i * cap + s] = particle index, in the walk order (the near
-// part and the reversed far part of the stored row merged back)
-__global__ void k_vl_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// Verlet list in the reference layout: out[i * cap + s] = particle index, in the walk order. One
+// wavefront per row: the walk keys of the row's entries (near part and far part) go to LDS, and each
+// entry's rank among them (keys are distinct) is its position.
+__global__ __launch_bounds__(WAVE) void k_vl_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
+  extern __shared__ uint64_t vl_keys[];
+  const uint32_t t = blockIdx.x, lane = threadIdx.x;
   if (t >= c.n) return;
   const uint32_t cap = c.list_cap, i = b.id[t], ct = b.cell_of[t];
   const uint32_t* row = b.vl + (size_t)t * vl_stride(c);
   const uint32_t cnt = b.vcount[t], cn = b.vnear[t];
-  uint32_t pn = 0, pf = cn;  // next near entry, next far entry (as row_index positions)
-  for (uint32_t s = 0; s < cap; ++s) {
-    uint32_t v = SOUP_NONE;
-    if (s < cnt) {
-      const bool take_near =
-          pf >= cnt || (pn < cn && walk_key(c, b.cell_of, ct, row[pn]) < walk_key(c, b.cell_of, ct, row[row_index(pf, cn, cap)]));
-      v = take_near ? row[pn++] : row[row_index(pf++, cn, cap)];
-      v = b.id[v];
-    }
-    out[(size_t)i * cap + s] = v;
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    const uint32_t u = row[row_index(s, cn, cap)];
+    vl_keys[s] = walk_key(c, b.cell_of, ct, u, b.id[u]);
   }
+  __syncthreads();
+  for (uint32_t s = lane; s < cnt; s += WAVE) {
+    const uint64_t me = vl_keys[s];
+    uint32_t rk = 0;
+    for (uint32_t q = 0; q < cnt; ++q) rk += vl_keys[q] < me ? 1u : 0u;
+    out[(size_t)i * cap + rk] = (uint32_t)me;
+  }
+  for (uint32_t s = cnt + lane; s < cap; s += WAVE) out[(size_t)i * cap + s] = SOUP_NONE;
+}
+// the reference's cell-sorted index list (CELL_IDX): the particle indices of each cell in ascending
+// order, from the slots, which order a cell by x
+__global__ __launch_bounds__(256) void k_cell_idx_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.n) return;
+  const uint32_t cell = b.cell_of[t], me = b.id[t];
+  const uint32_t s0 = b.cell_start[cell], s1 = b.cell_start[cell + 1u];
+  uint32_t r = 0;
+  for (uint32_t k = s0; k < s1; ++k) r += b.id[k] < me ? 1u : 0u;
+  out[s0 + r] = me;
 }
 // DH list in the reference layout: entries are head indices, exported as particle indices
 __global__ void k_es_to_index(const uint32_t* es, const uint32_t* es_count, const uint32_t* head_id, uint32_t* out,

@@ -281,8 +281,10 @@ static void ph_rebuild(SoupHandle* h) {
   fill(h, h->b.cell_count, 0u, h->ncells);
   hipLaunchKernelGGL(k_cell_count, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
   scan(h, h->b.cell_count, h->ncells, h->b.cell_start);
-  hipLaunchKernelGGL(k_cell_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
-  hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->b.cell_rank);
+  // the sort keys go to the alternate velocity buffer, which the permute overwrites next
+  uint2* keys = reinterpret_cast<uint2*>(h->alt.vel);
+  hipLaunchKernelGGL(k_cell_fill, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, keys);
+  hipLaunchKernelGGL(k_cell_rank, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, keys, h->b.cell_rank);
   hipLaunchKernelGGL(k_permute, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b, h->alt, h->b.cell_rank);
   std::swap(h->b.pos, h->alt.pos);
   std::swap(h->b.vel, h->alt.vel);
@@ -757,7 +759,6 @@ int soup_create(const SoupConfig* cfg, const SoupInitState* init, SoupHandle** o
         h->ncells_cap = h->ncells;
         DA(h, b.cell_rank, n);
         DA(h, b.cell_tmp, n);
-        DA(h, b.perm, n);
         DA(h, b.vl, (size_t)n * vl_stride(*cfg));
         DA(h, b.vcount, n);
         DA(h, b.vnear, n);
@@ -1561,7 +1562,13 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
                          reinterpret_cast<uint32_t*>(h->stage), n);
       break;
     case SOUP_FIELD_VERLET_LIST:
-      hipLaunchKernelGGL(k_vl_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b,
+      if ((size_t)h->c.list_cap * 8u > 160u * 1024u)
+        return fail(h, SOUP_E_INVALID, fmt("Verlet list readback: listCapacity=%u exceeds the 20480 entries one "
+                                       "workgroup's LDS can rank", h->c.list_cap));
+      if ((size_t)h->c.list_cap * 8u > 64u * 1024u)
+        HC(h, hipFuncSetAttribute((const void*)k_vl_to_index, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)((size_t)h->c.list_cap * 8u)));
+      hipLaunchKernelGGL(k_vl_to_index, dim3(n), dim3(WAVE), (size_t)h->c.list_cap * 8u, h->stream, h->c, h->b,
                          reinterpret_cast<uint32_t*>(h->stage));
       break;
     case SOUP_FIELD_ES_LIST:
@@ -1570,7 +1577,10 @@ int soup_read(SoupHandle* h, int field, void* dst, size_t bytes) {
       break;
     case SOUP_FIELD_ES_COUNT: src = h->b.es_count; break;
     case SOUP_FIELD_HEAD_IDX: src = h->b.head_id; break;
-    case SOUP_FIELD_CELL_IDX: src = h->b.id; break;  // slot order is the cell-sorted order
+    case SOUP_FIELD_CELL_IDX:
+      hipLaunchKernelGGL(k_cell_idx_to_index, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b,
+                         reinterpret_cast<uint32_t*>(h->stage));
+      break;
     case SOUP_FIELD_CELL_START: src = h->b.cell_start; break;
     case SOUP_FIELD_EVENTS: src = h->b.events; break;
     case SOUP_FIELD_DESORB: src = h->b.desorb; break;

@@ -71,12 +71,12 @@ __global__ __launch_bounds__(256) void kp_cell_count(const RepDesc* __restrict__
 __global__ __launch_bounds__(256) void kp_cell_fill(const RepDesc* __restrict__ reps, const uint32_t* __restrict__ off,
                                                     uint32_t nrep) {
   PACK_BEGIN(off);
-  cell_fill_body(d_->c, d_->b, lb_);
+  cell_fill_body(d_->c, d_->b, reinterpret_cast<uint2*>(d_->alt.vel), lb_);  // alt.vel is free until the permute
 }
 __global__ __launch_bounds__(256) void kp_cell_rank(const RepDesc* __restrict__ reps, const uint32_t* __restrict__ off,
                                                     uint32_t nrep) {
   PACK_BEGIN(off);
-  cell_rank_body(d_->c, d_->b, d_->b.cell_rank, lb_);
+  cell_rank_body(d_->c, d_->b, reinterpret_cast<const uint2*>(d_->alt.vel), d_->b.cell_rank, lb_);
 }
 __global__ __launch_bounds__(256) void kp_permute(const RepDesc* __restrict__ reps, const uint32_t* __restrict__ off,
                                                   uint32_t nrep) {
