@@ -121,6 +121,30 @@ __device__ __forceinline__ void scan_blocks_body(uint32_t* block_sums, uint32_t 
 }
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_blocks(uint32_t* block_sums, uint32_t nb) { scan_blocks_body(block_sums, nb, blockIdx.x); }
 
+// the whole exclusive scan in one workgroup, for m <= SCAN_ONE_MAX: the tiles in order, each a
+// coalesced block scan offset by the running total; out[m] = total
+#define SCAN_ONE_MAX (SCAN_TILE * 8u)
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_one(const uint32_t* in, uint32_t m, uint32_t* out) {
+  __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
+  uint32_t run = 0;
+  for (uint32_t t0 = 0; t0 < m; t0 += SCAN_TILE) {
+    const uint32_t base = t0 + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS], sum = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      v[k] = base + k < m ? in[base + k] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan(sum, s_wave, total) + run;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      if (base + k < m) out[base + k] = off;
+      off += v[k];
+    }
+    run += total;
+  }
+  if (threadIdx.x == 0) out[m] = run;
+}
+
 __device__ __forceinline__ void scan_apply_body(const uint32_t* in, uint32_t m, const uint32_t* block_sums, uint32_t nb, uint32_t* out, const uint32_t blk) {
   __shared__ uint32_t s_wave[SCAN_BLOCK / WAVE];
   uint32_t base = blk * SCAN_TILE + threadIdx.x * SCAN_ITEMS, v[SCAN_ITEMS], sum = 0;
@@ -398,6 +422,11 @@ __device__ __forceinline__ void permute_body(SoupConfig c, DevBuf b, DevBuf o, c
   o.id[ns] = id;
   o.cell_of[ns] = b.cell_tmp[t];
   b.inv[id] = ns;
+  // the head marks of the new order start clear (the DH build sets them), and the drift maximum
+  // restarts from zero
+  o.head_of_slot[ns] = SOUP_NONE;
+  o.head_flag[ns] = 0u;
+  if (t == 0u) b.flags[2] = 0u;
 }
 __global__ __launch_bounds__(256) void k_permute(SoupConfig c, DevBuf b, DevBuf o, const uint32_t* new_slot_of_old) { permute_body(c, b, o, new_slot_of_old, blockIdx.x); }
 

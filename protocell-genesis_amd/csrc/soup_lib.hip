@@ -202,6 +202,10 @@ static void fill(SoupHandle* h, uint32_t* p, uint32_t v, uint32_t count) {
 
 // exclusive scan of m u32 -> out[0..m]; block_sums holds SCAN_TILE + 1 entries
 static void scan_on(hipStream_t st, uint32_t* block_sums, const uint32_t* in, uint32_t m, uint32_t* out) {
+  if (m <= SCAN_ONE_MAX) {  // one launch instead of three
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN_BLOCK), 0, st, in, m, out);
+    return;
+  }
   uint32_t nb = blocks(m, SCAN_TILE);
   if (nb == 0) nb = 1;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, m, block_sums);
@@ -300,8 +304,7 @@ static void ph_rebuild(SoupHandle* h) {
   prof_end(h, SOUP_PROF_SORT, e0);
   if (es_on(h)) {
     hipEvent_t e1 = prof_begin(h, SOUP_PROF_ES_BUILD);
-    fill(h, h->b.head_of_slot, SOUP_NONE, n);
-    fill(h, h->b.head_flag, 0u, n);
+    // head_of_slot and head_flag were cleared by the permutation
     hipLaunchKernelGGL(k_head_slots, dim3(blocks(h->nh, 256)), dim3(256), 0, h->stream, h->b, h->nh);
     scan(h, h->b.head_flag, n, h->b.head_pos);
     hipLaunchKernelGGL(k_head_rows, dim3(blocks(n, 256)), dim3(256), 0, h->stream, h->c, h->b);
@@ -327,8 +330,7 @@ static void ph_rebuild(SoupHandle* h) {
   } else {
     hipLaunchKernelGGL(k_verlet_build, dim3(xcd_blocks((uint64_t)n * WAVE, 256)), dim3(256), 0, h->stream, h->c, h->b);
   }
-  prof_end(h, SOUP_PROF_VERLET_BUILD, e2);
-  fill(h, h->b.flags + 2, 0u, 1);
+  prof_end(h, SOUP_PROF_VERLET_BUILD, e2);  // the permutation restarted the drift maximum
 }
 
 // far DH rows of the charged heads (FP64 per head), then the list force, which adds them to the
