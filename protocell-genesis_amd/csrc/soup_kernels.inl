@@ -3195,7 +3195,8 @@ __global__ void k_bs_from_index(const uint32_t* src, const uint32_t* id, uint4* 
 }
 // Verlet list in the reference layout: out[i * cap + s] = particle index, in the walk order. One
 // wavefront per row: the walk keys of the row's entries (near part and far part) go to LDS, and each
-// entry's rank among them (keys are distinct) is its position.
+// entry's rank among them is its position. Equal keys (a cell the stencil reaches at two offsets on
+// a grid narrower than the stencil) keep their order in the stored row.
 __global__ __launch_bounds__(WAVE) void k_vl_to_index(SoupConfig c, DevBuf b, uint32_t* out) {
   extern __shared__ uint64_t vl_keys[];
   const uint32_t t = blockIdx.x, lane = threadIdx.x;
@@ -3211,7 +3212,7 @@ __global__ __launch_bounds__(WAVE) void k_vl_to_index(SoupConfig c, DevBuf b, ui
   for (uint32_t s = lane; s < cnt; s += WAVE) {
     const uint64_t me = vl_keys[s];
     uint32_t rk = 0;
-    for (uint32_t q = 0; q < cnt; ++q) rk += vl_keys[q] < me ? 1u : 0u;
+    for (uint32_t q = 0; q < cnt; ++q) rk += (vl_keys[q] < me || (vl_keys[q] == me && q < s)) ? 1u : 0u;
     out[(size_t)i * cap + rk] = (uint32_t)me;
   }
   for (uint32_t s = cnt + lane; s < cap; s += WAVE) out[(size_t)i * cap + s] = SOUP_NONE;
